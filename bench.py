@@ -1,0 +1,175 @@
+"""Benchmark: fused on-the-fly cost + LSE (LAMP mirror prox) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A step is one LAMP solve to L1 infeasibility 1e-6 (stop rule "infeas",
+epsilon = 6e-6) of the 64x64-grid workload (n = m = 4096, p = 2, eta_p = 1e-2,
+half Uniform / half Gaussian-bump histograms on [0,1]^2), in mixed precision
+(FP32 cost/LSE, FP64 potentials and sums), with inputs resident in HBM.
+
+Printed as one JSON line (rank 0):
+  value        dense-equivalent cost-LSE GB/s over the timed solves:
+               pair evaluations x 4 B (the FP32 cost matrix a dense path would
+               stream) + the O(n + m) vector bytes of each pass, / time;
+  roofline     the mixed-precision row-LSE kernel timed with HIP events on the
+               library's stream, bytes per launch / average launch time,
+               against 8 TB/s;
+  cpu_baseline the FP64 oracle LAMP (restatement of the reference loop) on a
+               bounded sample of the same workload on the host cores.
+Extra fields: solves/s and iterations/s for mixed and for the plain FP64
+single-instance path (host-checked stop every `frequency` iterations), and
+the mixed-vs-FP64 error of the transport cost.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "culamp.jl_amd"))
+
+METRIC = "cost-LSE GB/s + solves/s to 1e-6 infeasibility, batched/diff vs plain, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def pass_bytes(n: int, m: int, b_c: int, b_x: int = 8) -> float:
+    """Algorithmic bytes of one O(n*m) pass: n*m*b_c + (n+m)*3*b_x + 8(n+m) + 8n."""
+    return n * m * b_c + (n + m) * 3 * b_x + 8 * (n + m) + 8 * n
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--eta", type=float, default=1e-2)
+    ap.add_argument("--frequency", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--kernel-launches", type=int, default=200)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
+              "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import culamp_amd
+    from culamp_amd import workloads
+
+    def log(*x):
+        print(*x, file=sys.stderr, flush=True)
+
+    ctx = culamp_amd.Context(0)
+    g = a.grid
+    x, y, r, c = workloads.grid_histograms(g)
+    n, m = len(x), len(y)
+    pb = culamp_amd.Problem(x, y, r, c, ctx)
+    mixed = culamp_amd.EOTArgs(eta_p=a.eta, epsilon=6e-6, B=1.1, itermax=200000, verbose=False,
+                               stop="infeas", precision="mixed")
+    fp64 = mixed.copy(precision="fp64")
+
+    for _ in range(a.warmup):
+        pb.lamp(mixed, a.frequency, 2.0, log=None, assignments=False)
+    ctx.synchronize()
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        nu, phi, psi, _, _, info = pb.lamp(mixed, a.frequency, 2.0, log=None, assignments=False)
+        infos.append(info)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    passes = sum(i["passes"] for i in infos)
+    pairs = sum(i["pairs"] for i in infos)
+    iters = sum(i["iterations"] for i in infos)
+    # every pass of the LAMP loop is n x m (rows) or m x n (columns)
+    gbytes = (pairs * 4 + passes * (pass_bytes(n, m, 0))) / 1e9
+    value = gbytes / dt
+    log(f"mixed: {a.steps} solves in {dt:.3f}s, {iters / a.steps:.0f} it/solve, "
+        f"infeas {infos[-1]['residual']:.3e}, converged {infos[-1]['converged']}")
+
+    out_fp64 = {}
+    if not a.no_fp64:
+        t1 = time.perf_counter()
+        _, _, _, _, _, i64 = pb.lamp(fp64, 100, 2.0, log=None, assignments=False)
+        d64 = time.perf_counter() - t1
+        out_fp64 = {
+            "fp64_plain_solves_per_s": 1.0 / d64,
+            "fp64_plain_iterations_per_s": i64["iterations"] / d64,
+            "fp64_plain_iterations": i64["iterations"],
+            "fp64_plain_gbs_dense_eq": (i64["pairs"] * 8 + i64["passes"] * pass_bytes(n, m, 0)) / 1e9 / d64,
+            "mixed_vs_fp64_objective_rel_err": abs(infos[-1]["objective"] - i64["objective"]) / abs(i64["objective"]),
+            "speedup_mixed_vs_fp64_plain_solves": (a.steps / dt) * d64,
+        }
+        log(f"fp64 plain: {d64:.3f}s, {i64['iterations']} it, infeas {i64['residual']:.3e}")
+
+    # ---- roofline of the dominant kernel: mixed row LSE, events on the library stream
+    from culamp_amd import kernels as K
+    dx = culamp_amd.DeviceArray.from_host(ctx, x)
+    dy = culamp_amd.DeviceArray.from_host(ctx, y)
+    dth = culamp_amd.DeviceArray.from_host(ctx, nu)
+    dout = culamp_amd.DeviceArray(ctx, n)
+    reg = a.eta / 4.0  # eta_t at convergence = eta_p / (2 W_inf), W_inf = 2 on the unit grid
+    for _ in range(5):
+        K.warp_logsumexp_spp_ct_fused(ctx, dout, dx, dy, dth, reg, 1.0, 2.0, 2.0, culamp_amd.MIXED)
+    ctx.timer_start()
+    for _ in range(a.kernel_launches):
+        K.warp_logsumexp_spp_ct_fused(ctx, dout, dx, dy, dth, reg, 1.0, 2.0, 2.0, culamp_amd.MIXED)
+    ms = ctx.timer_stop() / a.kernel_launches
+    bpl = pass_bytes(n, m, 4)
+    achieved = bpl / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "pair_pass<mixed, LSE, sqL2> (culamp_lse_rows)",
+                "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": bpl,
+                "pairs_per_s": n * m / (ms * 1e-3)}
+    log(f"row-LSE kernel: {ms * 1e3:.2f} us/launch, {achieved:.0f} GB/s dense-eq")
+
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # the checker, timed here as the CPU baseline
+        threads = min(os.cpu_count() or 1, 16)
+        oracle.set_threads(threads)
+        tc = time.perf_counter()
+        o = oracle.lamp_solve(x, y, r, c, frequency=a.frequency, p=2.0, eta_p=a.eta, epsilon=6e-6,
+                              B=1.1, itermax=200000, verbose=False, stop="infeas",
+                              tmax=a.cpu_seconds)
+        dc = time.perf_counter() - tc
+        it_c = max(o["iterations"], 1)
+        # passes: 4 per iteration + 3 per check every `frequency` iterations
+        checks = (it_c + a.frequency - 1) // a.frequency
+        cpu_pairs = (4 * it_c + 3 * checks) * n * m
+        cpu_passes = 4 * it_c + 3 * checks
+        cpu = {"value": round((cpu_pairs * 8 + cpu_passes * pass_bytes(n, m, 0)) / 1e9 / dc, 3),
+               "unit": "GB/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle LAMP on the same {g}x{g} instance, {it_c} iterations "
+                         f"(tmax {a.cpu_seconds:.0f}s), dense-eq bytes at 8 B/pair",
+               "iterations_per_s": it_c / dc, "pairs_per_s": cpu_pairs / dc}
+        log(f"cpu oracle: {it_c} it in {dc:.2f}s")
+
+    ms_per_step = dt / a.steps * 1e3
+    rec = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"LAMP mirror prox, on-the-fly sq-Euclidean cost, {g}x{g} grid "
+                               f"histograms on [0,1]^2 (n=m={n}), eta_p={a.eta}, stop at L1 "
+                               f"infeasibility 1e-6, FP32 cost/LSE + FP64 potentials",
+                   "n": n, "m": m, "p": 2, "eta_p": a.eta, "frequency": a.frequency},
+        "solves_per_s": a.steps / dt, "iterations_per_s": iters / dt,
+        "iterations_per_solve": iters / a.steps, "final_infeasibility": infos[-1]["residual"],
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    rec.update(out_fp64)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,949 @@
+// culamp_api.hip -- host side of libculamp: contexts, device-resident
+// problems, the kernel-level C ABI and the solver loops (LAMP mirror prox,
+// Sinkhorn, annealed Sinkhorn) that drive the pair-pass kernels.
+//
+// The solver loops restate the reference's Julia host loops
+// (src/solvers/MirrorProxKernel.jl:1019-1132, src/solvers/SinkhornKernel.jl:
+// 477-559, src/solvers/AnnealedSinkhorn.jl:65-177) with the same stop tests,
+// log lines and outputs.  Unlike the reference they do not synchronise the
+// device every iteration: launches are queued on the context's stream and the
+// host only waits on iterations that log or test for convergence.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "culamp.h"
+#include "culamp_internal.h"
+
+using namespace culamp;
+
+struct culamp_ctx {
+  int device = 0;
+  int num_cu = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double* partial = nullptr;
+  size_t partial_cap = 0;  // doubles
+  unsigned* counters = nullptr;
+  size_t counter_cap = 0;
+  double* scal = nullptr;      // device scalars for reductions
+  double* scal_host = nullptr; // pinned mirror
+  int force_rpt = 0, force_splits = 0;
+  double pairs = 0.0;          // pair evaluations issued (throughput accounting)
+  int64_t passes = 0;
+};
+
+struct culamp_problem {
+  culamp_ctx* ctx = nullptr;
+  int64_t n = 0, m = 0;
+  double *x = nullptr, *y = nullptr, *r = nullptr, *c = nullptr;
+  double *logr = nullptr, *logc = nullptr;
+  std::vector<double> hr, hc;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(CULAMP_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define TRY(expr)                      \
+  do {                                 \
+    int rc_ = (expr);                  \
+    if (rc_ != CULAMP_OK) return rc_;  \
+  } while (0)
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Julia's @printf prints non-finite values as Inf/-Inf/NaN.
+std::string fmt(const char* f, double v) {
+  if (std::isnan(v)) return "NaN";
+  if (std::isinf(v)) return v > 0 ? "Inf" : "-Inf";
+  char b[64];
+  snprintf(b, sizeof b, f, v);
+  return b;
+}
+
+void emit(culamp_log_fn fn, void* user, const std::string& s) {
+  if (fn) fn(user, s.c_str());
+}
+
+int ensure_work(culamp_ctx* ctx, size_t partial_doubles, size_t counters) {
+  if (partial_doubles > ctx->partial_cap) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->partial) HIP_TRY(hipFree(ctx->partial));
+    ctx->partial = nullptr;
+    size_t cap = partial_doubles + partial_doubles / 2;
+    HIP_TRY(hipMalloc(&ctx->partial, cap * sizeof(double)));
+    ctx->partial_cap = cap;
+  }
+  if (counters > ctx->counter_cap) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->counters) HIP_TRY(hipFree(ctx->counters));
+    ctx->counters = nullptr;
+    size_t cap = counters + 1024;
+    HIP_TRY(hipMalloc(&ctx->counters, cap * sizeof(unsigned)));
+    HIP_TRY(hipMemsetAsync(ctx->counters, 0, cap * sizeof(unsigned), ctx->stream));
+    ctx->counter_cap = cap;
+  }
+  return CULAMP_OK;
+}
+
+int run_pass(culamp_ctx* ctx, const PassDesc& d, int precision, int kind) {
+  if (d.nt <= 0) return CULAMP_OK;
+  if (precision != CULAMP_FP64 && precision != CULAMP_MIXED)
+    return fail(CULAMP_ERR_INVALID_ARG, "precision must be CULAMP_FP64 or CULAMP_MIXED");
+  LaunchCfg cfg = choose_launch(d.nt, d.ns, precision, kind, ctx->num_cu,
+                                ctx->force_rpt, ctx->force_splits);
+  if (cfg.splits > 1) {
+    const size_t ntp = (size_t)cfg.tblocks * kThreads * cfg.rpt;
+    TRY(ensure_work(ctx, (size_t)pass_fields(kind) * cfg.splits * ntp, (size_t)cfg.tblocks));
+  }
+  Work w{ctx->partial, ctx->counters};
+  HIP_TRY(launch_pass(d, precision, kind, cfg, w, ctx->stream));
+  ctx->pairs += (double)d.nt * (double)d.ns;
+  ctx->passes += 1;
+  return CULAMP_OK;
+}
+
+PassDesc blank() {
+  PassDesc d;
+  std::memset(&d, 0, sizeof d);
+  return d;
+}
+
+int check_ctx(culamp_ctx* ctx) {
+  if (!ctx) return fail(CULAMP_ERR_INVALID_ARG, "null context");
+  HIP_TRY(hipSetDevice(ctx->device));
+  return CULAMP_OK;
+}
+
+int check_sizes(int64_t n, int64_t m) {
+  if (n < 0 || m < 0) return fail(CULAMP_ERR_INVALID_ARG, "negative size (n=%lld, m=%lld)", (long long)n, (long long)m);
+  return CULAMP_OK;
+}
+
+// ---- pass builders (kernel-level semantics) ---------------------------------
+
+// MirrorProxKernel.jl:833-938: v = -(C*st/(2W) + theta_j)/reg
+PassDesc lse_rows_desc(const double* x, int64_t n, const double* y, int64_t m,
+                       const double* theta, double reg, double st, double winf,
+                       double p, double* out) {
+  PassDesc d = blank();
+  const double c1 = st / (2.0 * winf);
+  const double invreg = 1.0 / reg;
+  d.tgt = x; d.nt = n; d.src = y; d.ns = m; d.p = p;
+  d.k = -c1 * invreg;
+  d.srcA = theta; d.alpha = -invreg;
+  d.epi = EPI_LSE; d.out0 = out;
+  return d;
+}
+
+// MirrorProxKernel.jl:7-94
+PassDesc colmarg_desc(const double* x, int64_t n, const double* y, int64_t m,
+                      const double* r, const double* theta, const double* logZ,
+                      double reg, double st, double winf, double p, double* res,
+                      double* cost) {
+  PassDesc d = blank();
+  const double c1 = st / (2.0 * winf);
+  const double invreg = 1.0 / reg;
+  d.tgt = y; d.nt = m; d.src = x; d.ns = n; d.p = p;
+  d.k = -c1 * invreg;
+  d.srcA = logZ; d.alpha = -1.0;
+  d.srcB = r;
+  d.tgtA = theta; d.gamma = -invreg;
+  d.epi = EPI_SUM; d.out0 = res; d.out1 = cost;
+  return d;
+}
+
+// SinkhornKernel.jl:341-444: out_i = log(marg_i) - LSE_j(-C/(reg W) + psi_j)
+PassDesc sk_half_desc(const double* x, int64_t n, const double* y, int64_t m,
+                      const double* marg, const double* psi, double reg,
+                      double winf, double p, double* out) {
+  PassDesc d = blank();
+  d.tgt = x; d.nt = n; d.src = y; d.ns = m; d.p = p;
+  d.k = -1.0 / (reg * winf);
+  d.srcA = psi; d.alpha = 1.0;
+  d.epi = EPI_SK_HALF; d.out0 = out; d.marg = marg;
+  return d;
+}
+
+// SinkhornKernel.jl:9-81
+PassDesc sk_resid_desc(const double* x, int64_t n, const double* y, int64_t m,
+                       const double* marg, const double* phi, const double* psi,
+                       double reg, double winf, double p, double* res,
+                       double* cost) {
+  PassDesc d = blank();
+  d.tgt = x; d.nt = n; d.src = y; d.ns = m; d.p = p;
+  d.k = -1.0 / (reg * winf);
+  d.srcA = psi; d.alpha = 1.0;
+  d.tgtA = phi; d.gamma = 1.0;
+  d.epi = EPI_SK_RESID; d.out0 = res; d.out1 = cost; d.marg = marg;
+  return d;
+}
+
+PassDesc max_desc(const double* x, int64_t n, const double* y, int64_t m,
+                  double p, double* out) {
+  PassDesc d = blank();
+  d.tgt = x; d.nt = n; d.src = y; d.ns = m; d.p = p;
+  d.epi = EPI_MAX; d.out0 = out;
+  return d;
+}
+
+PassDesc minplus_desc(const double* x, int64_t n, const double* y, int64_t m,
+                      const double* theta, double winf, double p, double* out) {
+  PassDesc d = blank();
+  d.tgt = x; d.nt = n; d.src = y; d.ns = m; d.p = p;
+  d.srcA = theta; d.alpha = 2.0 * winf;
+  d.epi = EPI_MIN; d.out0 = out;
+  return d;
+}
+
+// SinkhornKernel.jl:124-162
+PassDesc bary_sk_desc(const double* x, int64_t n, const double* y, int64_t m,
+                      const double* psi, double reg, double winf, double p,
+                      double* out) {
+  PassDesc d = blank();
+  d.tgt = x; d.nt = n; d.src = y; d.ns = m; d.p = p;
+  d.k = -1.0 / (reg * winf);
+  d.srcA = psi; d.alpha = 1.0;
+  d.epi = EPI_BARY; d.out0 = out;
+  return d;
+}
+
+// MirrorProxKernel.jl:97-142 (rows): weights exp(-(C/(2W) + nu_j)/reg)
+PassDesc bary_lamp_rows_desc(const double* x, int64_t n, const double* y,
+                             int64_t m, const double* nu, double reg,
+                             double winf, double p, double* out) {
+  PassDesc d = blank();
+  d.tgt = x; d.nt = n; d.src = y; d.ns = m; d.p = p;
+  d.k = -1.0 / (2.0 * winf * reg);
+  d.srcA = nu; d.alpha = -1.0 / reg;
+  d.epi = EPI_BARY; d.out0 = out;
+  return d;
+}
+
+// MirrorProxKernel.jl:144-189 (columns): weights r_i exp(-C/(2W reg) - logZ_i)
+PassDesc bary_lamp_cols_desc(const double* x, int64_t n, const double* y,
+                             int64_t m, const double* r, const double* logZ,
+                             double reg, double winf, double p, double* out) {
+  PassDesc d = blank();
+  d.tgt = y; d.nt = m; d.src = x; d.ns = n; d.p = p;
+  d.k = -1.0 / (2.0 * winf * reg);
+  d.srcA = logZ; d.alpha = -1.0;
+  d.srcB = r;
+  d.epi = EPI_BARY; d.out0 = out;
+  return d;
+}
+
+// W_inf = max_ij C_ij (MirrorProxKernel.jl:1032-1034): max pass, device max,
+// one scalar back to the host.
+int device_winf(culamp_ctx* ctx, const double* x, int64_t n, const double* y,
+                int64_t m, double p, double* tmp_n, double* winf) {
+  TRY(run_pass(ctx, max_desc(x, n, y, m, p, tmp_n), CULAMP_FP64, KIND_MAX));
+  ReduceSpec rs{};
+  rs.nops = 1; rs.op[0] = 3; rs.a[0] = tmp_n; rs.len[0] = n;
+  HIP_TRY(launch_reduce(rs, ctx->scal, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  *winf = ctx->scal_host[0];
+  return CULAMP_OK;
+}
+
+int reduce_to_host(culamp_ctx* ctx, const ReduceSpec& rs, double* out) {
+  HIP_TRY(launch_reduce(rs, ctx->scal, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(double) * rs.nops, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < rs.nops; ++k) out[k] = ctx->scal_host[k];
+  return CULAMP_OK;
+}
+
+// RAII bundle of device buffers for one solve.
+struct DevBufs {
+  std::vector<void*> ptrs;
+  ~DevBufs() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  int alloc(double** out, int64_t n) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (e != hipSuccess) return fail(CULAMP_ERR_ALLOC, "hipMalloc(%lld doubles): %s", (long long)n, hipGetErrorString(e));
+    ptrs.push_back(p);
+    *out = (double*)p;
+    return CULAMP_OK;
+  }
+};
+
+int copy_out(culamp_ctx* ctx, double* host, const double* dev, int64_t n) {
+  if (!host || n <= 0) return CULAMP_OK;
+  HIP_TRY(hipMemcpyAsync(host, dev, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  return CULAMP_OK;
+}
+
+// Sinkhorn kernels other than the fused LSE only know p == 1 or squared L2
+// (SinkhornKernel.jl:38-42, 259-263, 146-150).
+double sk_aux_p(double p) { return p == 1.0 ? 1.0 : 2.0; }
+
+struct SkState {
+  double* phi; double* psi; double* res; double* cost;
+};
+
+// One sinkhorn_color_transfer solve on device-resident state
+// (SinkhornKernel.jl:477-559); phi/psi hold the warm start on entry.
+int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
+                  int64_t frequency, double p, culamp_log_fn log_fn, void* user,
+                  SkState& st, culamp_result* out) {
+  const int64_t n = pb->n, m = pb->m;
+  const double t0 = now_s();
+  const double eta = a->eta_p;
+  double winf;
+  TRY(device_winf(ctx, pb->x, n, pb->y, m, p, st.res, &winf));
+  const double p_lse = (p == INFINITY) ? sk_aux_p(p) : p;
+  const double p_aux = sk_aux_p(p);
+  const int prec = a->precision;
+  if (a->verbose) emit(log_fn, user, "time(s),iter,infeas,ot_objective,dual\n");
+  int64_t num_iter = 0;
+  int converged = 0;
+  for (int64_t it = 1; it <= a->itermax; ++it) {
+    const double elapsed = now_s() - t0;
+    if (elapsed > a->tmax) break;
+    PassDesc d1 = sk_half_desc(pb->x, n, pb->y, m, pb->r, st.psi, eta, winf, p_lse, st.phi);
+    d1.prev_offset = 1;
+    TRY(run_pass(ctx, d1, prec, KIND_LSE));
+    PassDesc d2 = sk_half_desc(pb->y, m, pb->x, n, pb->c, st.phi, eta, winf, p_lse, st.psi);
+    d2.prev_offset = 1;
+    TRY(run_pass(ctx, d2, prec, KIND_LSE));
+    if ((it - 1) % frequency == 0) {
+      TRY(run_pass(ctx, sk_resid_desc(pb->x, n, pb->y, m, pb->r, st.phi, st.psi, eta, winf, p_aux, st.res, st.cost), prec, KIND_SUM));
+      ReduceSpec rs{};
+      rs.nops = 4;
+      rs.op[0] = 5; rs.a[0] = st.res; rs.len[0] = n;
+      rs.op[1] = 0; rs.a[1] = st.cost; rs.len[1] = n;
+      rs.op[2] = 2; rs.a[2] = st.psi; rs.b[2] = pb->c; rs.len[2] = m;
+      rs.op[3] = 2; rs.a[3] = st.phi; rs.b[3] = pb->r; rs.len[3] = n;
+      double v[4];
+      TRY(reduce_to_host(ctx, rs, v));
+      const double rr = v[0];
+      if (a->verbose) {
+        emit(log_fn, user, fmt("%.6e", elapsed) + "," + std::to_string(it) + "," +
+                               fmt("%.14e", rr) + "," + fmt("%.14e", v[1]) + "," +
+                               fmt("%.14e", v[2] + v[3]) + ",sinkhorn_kernel\n");
+      }
+      if (rr <= a->epsilon / 6.0) {
+        converged = 1;
+        break;
+      }
+    }
+    num_iter += 1;
+  }
+  TRY(run_pass(ctx, sk_resid_desc(pb->x, n, pb->y, m, pb->r, st.phi, st.psi, eta, winf, p_aux, st.res, st.cost), prec, KIND_SUM));
+  ReduceSpec rs{};
+  rs.nops = 4;
+  rs.op[0] = 5; rs.a[0] = st.res; rs.len[0] = n;
+  rs.op[1] = 0; rs.a[1] = st.cost; rs.len[1] = n;
+  rs.op[2] = 2; rs.a[2] = st.psi; rs.b[2] = pb->c; rs.len[2] = m;
+  rs.op[3] = 2; rs.a[3] = st.phi; rs.b[3] = pb->r; rs.len[3] = n;
+  double v[4];
+  TRY(reduce_to_host(ctx, rs, v));
+  if (out) {
+    out->iterations = num_iter;
+    out->objective = v[1];
+    out->residual = v[0];
+    out->primal = NAN;
+    out->dual = v[2] + v[3];
+    out->winf = winf;
+    out->eta_t = eta;
+    out->converged = converged;
+    out->nonfinite = !(std::isfinite(v[0]) && std::isfinite(v[1]));
+    out->elapsed_s = now_s() - t0;
+  }
+  return CULAMP_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+int culamp_abi_version(void) { return CULAMP_ABI_VERSION; }
+
+const char* culamp_last_error(void) { return g_err.c_str(); }
+
+// EOTArgs defaults (src/Utilities.jl:18-31).
+void culamp_args_default(culamp_args* a) {
+  if (!a) return;
+  a->eta_p = 0.0; a->eta_mu = 0.0; a->tau_p = 1.0; a->tau_mu = 1.0;
+  a->alpha = 0.01; a->B = 1.0; a->epsilon = 1e-4; a->anneal_mult = 0.95;
+  a->itermax = 10000; a->inner_iter = 10; a->tmax = INFINITY;
+  a->verbose = 1; a->precision = CULAMP_FP64;
+  a->stop = CULAMP_STOP_REFERENCE; a->reserved = 0;
+}
+
+int culamp_device_count(int* count) {
+  if (!count) return fail(CULAMP_ERR_INVALID_ARG, "null count");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return CULAMP_OK;
+}
+
+int culamp_ctx_create(int device, culamp_ctx** out) {
+  if (!out) return fail(CULAMP_ERR_INVALID_ARG, "null output pointer");
+  *out = nullptr;
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0)
+    return fail(CULAMP_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= cnt)
+    return fail(CULAMP_ERR_INVALID_ARG, "device %d out of range [0, %d)", device, cnt);
+  culamp_ctx* ctx = new culamp_ctx();
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  hipDeviceProp_t prop;
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+  if (e == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+  if (e == hipSuccess) e = hipMalloc(&ctx->scal, 64 * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(&ctx->scal_host, 64 * sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    culamp_ctx_destroy(ctx);
+    return fail(CULAMP_ERR_HIP, "context creation failed: %s", hipGetErrorString(e));
+  }
+  *out = ctx;
+  return CULAMP_OK;
+}
+
+int culamp_ctx_destroy(culamp_ctx* ctx) {
+  if (!ctx) return CULAMP_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->partial) (void)hipFree(ctx->partial);
+  if (ctx->counters) (void)hipFree(ctx->counters);
+  if (ctx->scal) (void)hipFree(ctx->scal);
+  if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CULAMP_OK;
+}
+
+// Launch-shape override for tests and tuning: 0 = automatic.
+int culamp_ctx_set_launch(culamp_ctx* ctx, int rows_per_thread, int splits) {
+  if (!ctx) return fail(CULAMP_ERR_INVALID_ARG, "null context");
+  if (rows_per_thread != 0 && rows_per_thread != 1 && rows_per_thread != 2 && rows_per_thread != 4)
+    return fail(CULAMP_ERR_INVALID_ARG, "rows_per_thread must be 0, 1, 2 or 4");
+  if (splits < 0) return fail(CULAMP_ERR_INVALID_ARG, "splits must be >= 0");
+  ctx->force_rpt = rows_per_thread;
+  ctx->force_splits = splits;
+  return CULAMP_OK;
+}
+
+int culamp_ctx_synchronize(culamp_ctx* ctx) {
+  TRY(check_ctx(ctx));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return CULAMP_OK;
+}
+
+int culamp_malloc(culamp_ctx* ctx, size_t bytes, void** dptr) {
+  TRY(check_ctx(ctx));
+  if (!dptr) return fail(CULAMP_ERR_INVALID_ARG, "null output pointer");
+  hipError_t e = hipMalloc(dptr, bytes > 0 ? bytes : 1);
+  if (e != hipSuccess) return fail(CULAMP_ERR_ALLOC, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return CULAMP_OK;
+}
+
+int culamp_free(culamp_ctx* ctx, void* dptr) {
+  TRY(check_ctx(ctx));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (dptr) HIP_TRY(hipFree(dptr));
+  return CULAMP_OK;
+}
+
+int culamp_memcpy_htod(culamp_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  TRY(check_ctx(ctx));
+  if (bytes == 0) return CULAMP_OK;
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return CULAMP_OK;
+}
+
+int culamp_memcpy_dtoh(culamp_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  TRY(check_ctx(ctx));
+  if (bytes == 0) return CULAMP_OK;
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return CULAMP_OK;
+}
+
+int culamp_timer_start(culamp_ctx* ctx) {
+  TRY(check_ctx(ctx));
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  return CULAMP_OK;
+}
+
+int culamp_timer_stop(culamp_ctx* ctx, double* ms) {
+  TRY(check_ctx(ctx));
+  if (!ms) return fail(CULAMP_ERR_INVALID_ARG, "null output pointer");
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(hipEventSynchronize(ctx->ev1));
+  float f = 0.f;
+  HIP_TRY(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+  *ms = (double)f;
+  return CULAMP_OK;
+}
+
+// ---- kernel level -----------------------------------------------------------
+
+int culamp_max_cost(culamp_ctx* ctx, const double* x, int64_t n, const double* y,
+                    int64_t m, double p, double* out) {
+  TRY(check_ctx(ctx));
+  TRY(check_sizes(n, m));
+  return run_pass(ctx, max_desc(x, n, y, m, p, out), CULAMP_FP64, KIND_MAX);
+}
+
+int culamp_lse_rows(culamp_ctx* ctx, const double* x, int64_t n, const double* y,
+                    int64_t m, const double* theta, double reg, double st,
+                    double winf, double p, int precision, double* out) {
+  TRY(check_ctx(ctx));
+  TRY(check_sizes(n, m));
+  return run_pass(ctx, lse_rows_desc(x, n, y, m, theta, reg, st, winf, p, out), precision, KIND_LSE);
+}
+
+int culamp_col_marginal_cost(culamp_ctx* ctx, const double* x, int64_t n,
+                             const double* y, int64_t m, const double* r,
+                             const double* theta, const double* logZ, double reg,
+                             double st, double winf, double p, int precision,
+                             double* res, double* cost) {
+  TRY(check_ctx(ctx));
+  TRY(check_sizes(n, m));
+  return run_pass(ctx, colmarg_desc(x, n, y, m, r, theta, logZ, reg, st, winf, p, res, cost), precision, KIND_SUM);
+}
+
+int culamp_minplus_rows(culamp_ctx* ctx, const double* x, int64_t n,
+                        const double* y, int64_t m, const double* theta,
+                        double winf, double p, double* out) {
+  TRY(check_ctx(ctx));
+  TRY(check_sizes(n, m));
+  return run_pass(ctx, minplus_desc(x, n, y, m, theta, winf, p, out), CULAMP_FP64, KIND_MIN);
+}
+
+int culamp_lamp_theta_update(culamp_ctx* ctx, double* theta, const double* theta0,
+                             const double* res, const double* c,
+                             const double* eta_mu_i, int64_t m, double eta_mu,
+                             int adjust, double minv, double maxv, double winf) {
+  TRY(check_ctx(ctx));
+  TRY(check_sizes(m, 0));
+  HIP_TRY(launch_theta_update(theta, theta0, res, c, eta_mu_i, m, eta_mu, adjust, minv, maxv, winf, ctx->stream));
+  return CULAMP_OK;
+}
+
+int culamp_sinkhorn_half_step(culamp_ctx* ctx, const double* x, int64_t n,
+                              const double* y, int64_t m, const double* marg,
+                              const double* psi, double reg, double winf,
+                              double p, int precision, double* out) {
+  TRY(check_ctx(ctx));
+  TRY(check_sizes(n, m));
+  return run_pass(ctx, sk_half_desc(x, n, y, m, marg, psi, reg, winf, p, out), precision, KIND_LSE);
+}
+
+int culamp_row_residual_cost(culamp_ctx* ctx, const double* x, int64_t n,
+                             const double* y, int64_t m, const double* marg,
+                             const double* phi, const double* psi, double reg,
+                             double winf, double p, int precision, double* res,
+                             double* cost) {
+  TRY(check_ctx(ctx));
+  TRY(check_sizes(n, m));
+  return run_pass(ctx, sk_resid_desc(x, n, y, m, marg, phi, psi, reg, winf, p, res, cost), precision, KIND_SUM);
+}
+
+int culamp_barycentric_sinkhorn(culamp_ctx* ctx, const double* x, int64_t n,
+                                const double* y, int64_t m, const double* phi,
+                                const double* psi, double reg, double winf,
+                                double p, double* out) {
+  (void)phi;  // phi_i scales every weight of row i and cancels in the mean
+  TRY(check_ctx(ctx));
+  TRY(check_sizes(n, m));
+  return run_pass(ctx, bary_sk_desc(x, n, y, m, psi, reg, winf, p, out), CULAMP_FP64, KIND_BARY);
+}
+
+int culamp_barycentric_lamp_rows(culamp_ctx* ctx, const double* x, int64_t n,
+                                 const double* y, int64_t m, const double* nu,
+                                 const double* r, const double* logZ,
+                                 double reg, double winf, double p, double* out) {
+  (void)r; (void)logZ;  // per-row factors cancel in the mean
+  TRY(check_ctx(ctx));
+  TRY(check_sizes(n, m));
+  return run_pass(ctx, bary_lamp_rows_desc(x, n, y, m, nu, reg, winf, p, out), CULAMP_FP64, KIND_BARY);
+}
+
+int culamp_barycentric_lamp_cols(culamp_ctx* ctx, const double* x, int64_t n,
+                                 const double* y, int64_t m, const double* r,
+                                 const double* nu, const double* logZ,
+                                 double reg, double winf, double p, double* out) {
+  (void)nu;  // nu_j scales every weight of column j and cancels in the mean
+  TRY(check_ctx(ctx));
+  TRY(check_sizes(n, m));
+  return run_pass(ctx, bary_lamp_cols_desc(x, n, y, m, r, logZ, reg, winf, p, out), CULAMP_FP64, KIND_BARY);
+}
+
+// ---- problems ---------------------------------------------------------------
+
+int culamp_problem_create(culamp_ctx* ctx, const double* x, int64_t n,
+                          const double* y, int64_t m, const double* r,
+                          const double* c, culamp_problem** out) {
+  TRY(check_ctx(ctx));
+  if (!out) return fail(CULAMP_ERR_INVALID_ARG, "null output pointer");
+  *out = nullptr;
+  if (n <= 0 || m <= 0) return fail(CULAMP_ERR_INVALID_ARG, "problem sizes must be positive (n=%lld, m=%lld)", (long long)n, (long long)m);
+  if (!x || !y || !r || !c) return fail(CULAMP_ERR_INVALID_ARG, "null input array");
+  culamp_problem* pb = new culamp_problem();
+  pb->ctx = ctx; pb->n = n; pb->m = m;
+  pb->hr.assign(r, r + n);
+  pb->hc.assign(c, c + m);
+  hipError_t e = hipSuccess;
+  auto al = [&](double** p, int64_t len) { if (e == hipSuccess) e = hipMalloc(p, sizeof(double) * (size_t)len); };
+  al(&pb->x, 3 * n); al(&pb->y, 3 * m); al(&pb->r, n); al(&pb->c, m); al(&pb->logr, n); al(&pb->logc, m);
+  if (e == hipSuccess) e = hipMemcpyAsync(pb->x, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(pb->y, y, sizeof(double) * 3 * m, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(pb->r, r, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(pb->c, c, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_log(pb->logr, pb->r, n, ctx->stream);
+  if (e == hipSuccess) e = launch_log(pb->logc, pb->c, m, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    culamp_problem_destroy(pb);
+    return fail(CULAMP_ERR_HIP, "problem upload failed: %s", hipGetErrorString(e));
+  }
+  *out = pb;
+  return CULAMP_OK;
+}
+
+int culamp_problem_destroy(culamp_problem* pb) {
+  if (!pb) return CULAMP_OK;
+  (void)hipSetDevice(pb->ctx->device);
+  (void)hipStreamSynchronize(pb->ctx->stream);
+  for (double* p : {pb->x, pb->y, pb->r, pb->c, pb->logr, pb->logc})
+    if (p) (void)hipFree(p);
+  delete pb;
+  return CULAMP_OK;
+}
+
+// ---- solvers ----------------------------------------------------------------
+
+// extragradient_color_transfer (src/solvers/MirrorProxKernel.jl:1019-1132).
+int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
+                      int64_t frequency, double p, culamp_log_fn log_fn,
+                      void* user, double* nu_out, double* phi_out,
+                      double* psi_out, double* img1, double* img2,
+                      culamp_result* out) {
+  TRY(check_ctx(ctx));
+  if (!pb || !a) return fail(CULAMP_ERR_INVALID_ARG, "null problem or args");
+  if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
+  const int64_t n = pb->n, m = pb->m;
+  const int prec = a->precision;
+  const double pairs0 = ctx->pairs;
+  const int64_t passes0 = ctx->passes;
+  DevBufs B;
+  double *theta, *nu, *thetab, *nub, *res, *cost, *etamu, *logZ, *tmp;
+  TRY(B.alloc(&theta, m)); TRY(B.alloc(&nu, m)); TRY(B.alloc(&thetab, m));
+  TRY(B.alloc(&nub, m)); TRY(B.alloc(&res, m)); TRY(B.alloc(&cost, m));
+  TRY(B.alloc(&etamu, m)); TRY(B.alloc(&logZ, n)); TRY(B.alloc(&tmp, n > m ? n : m));
+  for (double* v : {theta, nu, thetab, nub, res, cost}) HIP_TRY(launch_fill(v, 0.0, m, ctx->stream));
+  HIP_TRY(launch_fill(logZ, 0.0, n, ctx->stream));
+  const double t0 = now_s();
+  double winf;
+  TRY(device_winf(ctx, pb->x, n, pb->y, m, p, tmp, &winf));
+  const double eta = a->eta_p / (2.0 * winf);                             // 1035
+  HIP_TRY(launch_etamu(etamu, pb->c, a->alpha / (double)n, a->tau_mu, m, ctx->stream));  // 1037
+  double negent = 0.0;
+  for (double v : pb->hr) if (v > 0.0) negent += v * std::log(v);
+  const double hr = eta * negent;                                         // 1066
+  emit(log_fn, user, "time(s),iter,infeas,ot_objective,primal,dual,solver\n");
+  double st = 1.0, etat = INFINITY;
+  const double taup = 1.0;  // hard-coded in the reference (1073)
+  const double minv = std::tanh(-a->B / 2.0), maxv = std::tanh(a->B / 2.0);
+  double primal = NAN, dual = NAN, resid = NAN, objective = NAN;
+  int converged = 0;
+  int64_t steps = 0;
+
+  auto lse = [&](const double* pot, double reg, double s, double* dst, bool offset) {
+    PassDesc d = lse_rows_desc(pb->x, n, pb->y, m, pot, reg, s, winf, p, dst);
+    d.prev_offset = offset ? 1 : 0;
+    return run_pass(ctx, d, prec, KIND_LSE);
+  };
+  for (int64_t it = 1; it <= a->itermax; ++it) {
+    const double elapsed = now_s() - t0;
+    if (elapsed > a->tmax) break;
+    const bool check_it = a->verbose || a->stop == CULAMP_STOP_INFEAS;
+    if (check_it && (it - 1) % frequency == 0) {  // 1081-1101
+      TRY(lse(nu, etat, st, logZ, true));
+      TRY(run_pass(ctx, colmarg_desc(pb->x, n, pb->y, m, pb->r, nu, logZ, etat, st, winf, p, res, cost), prec, KIND_SUM));
+      ReduceSpec rs{};
+      rs.nops = 4;
+      rs.op[0] = 1; rs.a[0] = res; rs.b[0] = pb->c; rs.len[0] = m;
+      rs.op[1] = 0; rs.a[1] = cost; rs.len[1] = m;
+      rs.op[2] = 2; rs.a[2] = res; rs.b[2] = nu; rs.len[2] = m;
+      rs.op[3] = 2; rs.a[3] = pb->r; rs.b[3] = logZ; rs.len[3] = n;
+      double v[4];
+      TRY(reduce_to_host(ctx, rs, v));
+      resid = v[0];
+      objective = v[1];
+      const double q = eta / etat;
+      primal = objective * (1.0 - q) - q * v[2] + q * v[3] + 2.0 * winf * resid;
+      ReduceSpec rd{};
+      rd.nops = 2;
+      if (eta > 0) {
+        TRY(lse(theta, eta, 1.0, tmp, false));
+      } else {
+        TRY(run_pass(ctx, minplus_desc(pb->x, n, pb->y, m, theta, winf, p, tmp), CULAMP_FP64, KIND_MIN));
+      }
+      rd.op[0] = 2; rd.a[0] = pb->r; rd.b[0] = tmp; rd.len[0] = n;
+      rd.op[1] = 2; rd.a[1] = pb->c; rd.b[1] = theta; rd.len[1] = m;
+      double w[2];
+      TRY(reduce_to_host(ctx, rd, w));
+      if (eta > 0) dual = -2.0 * eta * w[0] - 2.0 * winf * w[1] + 2.0 * hr;
+      else dual = w[0] - 2.0 * winf * w[1];
+      if (a->verbose)
+        emit(log_fn, user, fmt("%.6e", elapsed) + "," + std::to_string(it) + "," +
+                               fmt("%.14e", resid) + "," + fmt("%.14e", objective) + "," +
+                               fmt("%.14e", primal) + "," + fmt("%.14e", dual) + ",lamp_kernel\n");
+      const bool met = a->stop == CULAMP_STOP_INFEAS
+                           ? resid < a->epsilon / 6.0
+                           : (primal - dual < a->epsilon / 6.0 && resid < a->epsilon / 6.0);
+      if (met) {
+        converged = 1;
+        break;
+      }
+    }
+    // extragradient step (1103-1115); the theta update and the nu mixing are
+    // fused into the column-marginal pass epilogue.
+    const double etat_new = 1.0 / (taup + (1.0 / etat) * (1.0 - eta));
+    TRY(lse(nu, etat, st, logZ, true));
+    {
+      PassDesc d = colmarg_desc(pb->x, n, pb->y, m, pb->r, nu, logZ, etat, st, winf, p, res, cost);
+      d.epi = EPI_LAMP_COL; d.fuse_update = 1; d.adjust = 0;
+      d.th0 = theta; d.cvec = pb->c; d.etamu = etamu; d.eta_mu = a->eta_mu;
+      d.minv = minv; d.maxv = maxv; d.upd_winf = 1.0; d.th_out = thetab;
+      d.nu_in = nu; d.mix_src = theta; d.nu_out = nub; d.mix_t = etat_new;
+      TRY(run_pass(ctx, d, prec, KIND_SUM));
+    }
+    etat = etat_new;
+    st = (1.0 - etat) * st + etat;
+    TRY(lse(nub, etat, st, logZ, true));
+    {
+      PassDesc d = colmarg_desc(pb->x, n, pb->y, m, pb->r, nub, logZ, etat, st, winf, p, res, cost);
+      d.epi = EPI_LAMP_COL; d.fuse_update = 1; d.adjust = 1;
+      d.th0 = theta; d.cvec = pb->c; d.etamu = etamu; d.eta_mu = a->eta_mu;
+      d.minv = minv; d.maxv = maxv; d.upd_winf = 1.0; d.th_out = theta;
+      d.nu_in = nu; d.mix_src = thetab; d.nu_out = nu; d.mix_t = etat;
+      TRY(run_pass(ctx, d, prec, KIND_SUM));
+    }
+    steps += 1;
+  }
+  // epilogue (1124-1130)
+  TRY(lse(nu, etat, st, logZ, true));
+  double* img_dev = nullptr;
+  if (img1 || img2) TRY(B.alloc(&img_dev, 3 * (n > m ? n : m)));
+  if (img1) {
+    TRY(run_pass(ctx, bary_lamp_rows_desc(pb->x, n, pb->y, m, nu, etat, winf, p, img_dev), CULAMP_FP64, KIND_BARY));
+    TRY(copy_out(ctx, img1, img_dev, 3 * n));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  if (img2) {
+    TRY(run_pass(ctx, bary_lamp_cols_desc(pb->x, n, pb->y, m, pb->r, logZ, etat, winf, p, img_dev), CULAMP_FP64, KIND_BARY));
+    TRY(copy_out(ctx, img2, img_dev, 3 * m));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  if (psi_out) {
+    HIP_TRY(launch_axpby(tmp, nu, -2.0 * winf / a->eta_p, nu, 0.0, m, ctx->stream));
+    TRY(copy_out(ctx, psi_out, tmp, m));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  if (phi_out) {
+    HIP_TRY(launch_log_minus(tmp, pb->r, logZ, n, ctx->stream));
+    TRY(copy_out(ctx, phi_out, tmp, n));
+  }
+  TRY(copy_out(ctx, nu_out, nu, m));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (out) {
+    out->iterations = steps;
+    out->objective = objective;
+    out->residual = resid;
+    out->primal = primal;
+    out->dual = dual;
+    out->winf = winf;
+    out->eta_t = etat;
+    out->elapsed_s = now_s() - t0;
+    out->converged = converged;
+    out->pairs = ctx->pairs - pairs0;
+    out->passes = ctx->passes - passes0;
+    out->nonfinite = (a->verbose || a->stop == CULAMP_STOP_INFEAS) &&
+                     !(std::isfinite(resid) && std::isfinite(objective));
+  }
+  return CULAMP_OK;
+}
+
+int culamp_sinkhorn_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
+                          int64_t frequency, double p, const double* phi0,
+                          const double* psi0, culamp_log_fn log_fn, void* user,
+                          double* phi_out, double* psi_out, double* img1,
+                          double* img2, culamp_result* out) {
+  TRY(check_ctx(ctx));
+  if (!pb || !a) return fail(CULAMP_ERR_INVALID_ARG, "null problem or args");
+  if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
+  const int64_t n = pb->n, m = pb->m;
+  const double pairs0 = ctx->pairs;
+  const int64_t passes0 = ctx->passes;
+  DevBufs B;
+  SkState st;
+  TRY(B.alloc(&st.phi, n)); TRY(B.alloc(&st.psi, m));
+  TRY(B.alloc(&st.res, n > m ? n : m)); TRY(B.alloc(&st.cost, n));
+  if (phi0) HIP_TRY(hipMemcpyAsync(st.phi, phi0, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  else HIP_TRY(launch_fill(st.phi, 0.0, n, ctx->stream));
+  if (psi0) HIP_TRY(hipMemcpyAsync(st.psi, psi0, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+  else HIP_TRY(launch_fill(st.psi, 0.0, m, ctx->stream));
+  culamp_result r{};
+  TRY(sinkhorn_core(ctx, pb, a, frequency, p, log_fn, user, st, &r));
+  const double p_aux = sk_aux_p(p);
+  double* img_dev = nullptr;
+  if (img1 || img2) TRY(B.alloc(&img_dev, 3 * (n > m ? n : m)));
+  if (img1) {
+    TRY(run_pass(ctx, bary_sk_desc(pb->x, n, pb->y, m, st.psi, a->eta_p, r.winf, p_aux, img_dev), CULAMP_FP64, KIND_BARY));
+    TRY(copy_out(ctx, img1, img_dev, 3 * n));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  if (img2) {
+    TRY(run_pass(ctx, bary_sk_desc(pb->y, m, pb->x, n, st.phi, a->eta_p, r.winf, p_aux, img_dev), CULAMP_FP64, KIND_BARY));
+    TRY(copy_out(ctx, img2, img_dev, 3 * m));
+  }
+  TRY(copy_out(ctx, phi_out, st.phi, n));
+  TRY(copy_out(ctx, psi_out, st.psi, m));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  r.pairs = ctx->pairs - pairs0;
+  r.passes = ctx->passes - passes0;
+  if (out) *out = r;
+  return CULAMP_OK;
+}
+
+// annealed_sinkhorn_euclidean (src/solvers/AnnealedSinkhorn.jl:65-177).
+int culamp_annealed_sinkhorn_solve(culamp_ctx* ctx, culamp_problem* pb,
+                                   const culamp_args* a_in, int64_t frequency,
+                                   double p, culamp_log_fn log_fn, void* user,
+                                   double* phi_out, double* psi_out,
+                                   double* img1, double* img2,
+                                   culamp_result* out) {
+  TRY(check_ctx(ctx));
+  if (!pb || !a_in) return fail(CULAMP_ERR_INVALID_ARG, "null problem or args");
+  if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
+  const int64_t n = pb->n, m = pb->m;
+  culamp_args a = *a_in;
+  const double pairs0 = ctx->pairs;
+  const int64_t passes0 = ctx->passes;
+  const double eta_final = a.epsilon;
+  double eta = std::fmax(a.eta_p, eta_final);
+  const double t0 = now_s();
+  DevBufs B;
+  SkState st;
+  double *phiws, *psiws, *phiprev, *psiprev;
+  TRY(B.alloc(&st.phi, n)); TRY(B.alloc(&st.psi, m));
+  TRY(B.alloc(&st.res, n > m ? n : m)); TRY(B.alloc(&st.cost, n));
+  TRY(B.alloc(&phiws, n)); TRY(B.alloc(&psiws, m));
+  TRY(B.alloc(&phiprev, n)); TRY(B.alloc(&psiprev, m));
+  HIP_TRY(hipMemcpyAsync(phiws, pb->logr, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(psiws, pb->logc, sizeof(double) * m, hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(phiprev, pb->logr, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(psiprev, pb->logc, sizeof(double) * m, hipMemcpyDeviceToDevice, ctx->stream));
+  double objective = NAN, residual = NAN;
+  const double q = a.anneal_mult;
+  double preveta = 1.0;
+  const int verbose = a.verbose;
+  a.verbose = 0;
+  int64_t total = 0;
+  const double tmax = a.tmax;
+  culamp_result r{};
+  if (verbose) emit(log_fn, user, "time(s),iter,infeas,ot_objective,eta,solver\n");
+  for (;;) {
+    a.eta_p = eta;
+    HIP_TRY(hipMemcpyAsync(st.phi, phiws, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(st.psi, psiws, sizeof(double) * m, hipMemcpyDeviceToDevice, ctx->stream));
+    TRY(sinkhorn_core(ctx, pb, &a, frequency, p, nullptr, nullptr, st, &r));
+    total += r.iterations;
+    objective = r.objective;
+    residual = r.residual;
+    const double elapsed = now_s() - t0;
+    a.tmax = tmax - elapsed;
+    if (verbose)
+      emit(log_fn, user, fmt("%.6e", elapsed) + "," + std::to_string(total) + "," +
+                             fmt("%.14e", residual) + "," + fmt("%.14e", objective) + "," +
+                             fmt("%.14e", eta) + ",annealed_sinkhorn_kernel\n");
+    if (eta == eta_final || elapsed >= tmax) break;
+    const double f = eta / preveta;
+    // phi_ws = phi + f (phi - phi_prev); phi_prev = phi   (123-126)
+    HIP_TRY(launch_axpby(phiws, st.phi, 1.0 + f, phiprev, -f, n, ctx->stream));
+    HIP_TRY(launch_axpby(psiws, st.psi, 1.0 + f, psiprev, -f, m, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(phiprev, st.phi, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(psiprev, st.psi, sizeof(double) * m, hipMemcpyDeviceToDevice, ctx->stream));
+    preveta = eta;
+    eta = std::fmax(q * eta, eta_final);
+  }
+  double winf = r.winf;
+  const double p_aux = sk_aux_p(p);
+  double* img_dev = nullptr;
+  if (img1 || img2) TRY(B.alloc(&img_dev, 3 * (n > m ? n : m)));
+  if (img1) {
+    TRY(run_pass(ctx, bary_sk_desc(pb->x, n, pb->y, m, st.psi, eta_final, winf, p_aux, img_dev), CULAMP_FP64, KIND_BARY));
+    TRY(copy_out(ctx, img1, img_dev, 3 * n));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  if (img2) {
+    TRY(run_pass(ctx, bary_sk_desc(pb->y, m, pb->x, n, st.phi, eta_final, winf, p_aux, img_dev), CULAMP_FP64, KIND_BARY));
+    TRY(copy_out(ctx, img2, img_dev, 3 * m));
+  }
+  TRY(copy_out(ctx, phi_out, st.phi, n));
+  TRY(copy_out(ctx, psi_out, st.psi, m));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (out) {
+    out->iterations = total;
+    out->objective = objective;
+    out->residual = residual;
+    out->primal = NAN;
+    out->dual = NAN;
+    out->winf = winf;
+    out->eta_t = eta;
+    out->elapsed_s = now_s() - t0;
+    out->converged = residual <= eta_final / 6.0;
+    out->pairs = ctx->pairs - pairs0;
+    out->passes = ctx->passes - passes0;
+    out->nonfinite = !(std::isfinite(residual) && std::isfinite(objective));
+  }
+  return CULAMP_OK;
+}
+
+}  // extern "C"

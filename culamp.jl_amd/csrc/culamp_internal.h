@@ -1,0 +1,133 @@
+// culamp_internal.h -- types shared by the pair-pass kernels and the host side
+// of libculamp.  Not part of the public ABI (see include/culamp.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace culamp {
+
+// Cost kinds of the reference kernels' `l2dist`
+// (src/solvers/MirrorProxKernel.jl:869-878).
+enum Cost : int { COST_L1 = 0, COST_SQL2 = 1, COST_LINF = 2, COST_GENP = 3 };
+
+inline int cost_of_p(double p) {
+  if (p == 1.0) return COST_L1;
+  if (p == 2.0) return COST_SQL2;
+  if (p == __builtin_inf()) return COST_LINF;
+  return COST_GENP;
+}
+
+// What a pass reduces over the sources s for each target t, with
+//   v_ts = k * C(t, s) + alpha * srcA[s] + gamma * tgtA[t]
+//   KIND_LSE : log sum_s exp(v_ts)                       (max-shifted)
+//   KIND_SUM : sum_s srcB[s] exp(v_ts), sum_s srcB[s] exp(v_ts) C(t,s)
+//   KIND_BARY: sum_s srcB[s] exp(v_ts) pos_s / sum_s srcB[s] exp(v_ts)
+//   KIND_MAX : max_s C(t, s)
+//   KIND_MIN : min_s C(t, s) + alpha * srcA[s]
+enum Kind : int { KIND_LSE = 0, KIND_SUM = 1, KIND_BARY = 2, KIND_MAX = 3, KIND_MIN = 4 };
+
+// Per-target epilogue applied once the split partials are merged.
+enum Epi : int {
+  EPI_LSE = 0,       // out0[t] = LSE
+  EPI_SK_HALF = 1,   // out0[t] = log(marg[t]) - LSE
+  EPI_SUM = 2,       // out0[t] = S, out1[t] = Sc
+  EPI_SK_RESID = 3,  // out0[t] = marg[t] - S, out1[t] = Sc
+  EPI_LAMP_COL = 4,  // out0[t] = S, out1[t] = Sc, then the fused theta / nu update
+  EPI_BARY = 5,      // out0[3t + k] = weighted mean of source coordinate k
+  EPI_MAX = 6,       // out0[t] = max
+  EPI_MIN = 7        // out0[t] = min
+};
+
+struct PassDesc {
+  const double* tgt;  // 3 x nt, column-major (point-major)
+  const double* src;  // 3 x ns
+  int64_t nt, ns;
+  double p;           // exponent for COST_GENP
+  double k;           // cost scale in the exponent (natural-log units)
+  const double* srcA; double alpha;   // source exponent term (may be null)
+  const double* srcB;                 // source multiplicative weight (may be null)
+  const double* tgtA; double gamma;   // target exponent term (may be null)
+  int prev_offset;    // LSE kinds, mixed precision: centre each target on the
+                      // previous value of its output (see pair_pass)
+  int epi;
+  double* out0; double* out1;
+  const double* marg;
+  // EPI_LAMP_COL fused update (update_theta_residual, MirrorProx.jl:159-180,
+  // and the nu mixing of MirrorProxKernel.jl:1109/1115).
+  int fuse_update; int adjust;
+  const double* th0; const double* cvec; const double* etamu;
+  double eta_mu, minv, maxv, upd_winf;
+  double* th_out;
+  const double* nu_in; const double* mix_src; double* nu_out; double mix_t;
+};
+
+struct Work {
+  double* partial;     // split partials, [field][split][target]
+  unsigned* counters;  // one arrival counter per target block (self-resetting)
+};
+
+struct LaunchCfg {
+  int rpt;          // targets per thread
+  int splits;       // source splits (grid.y)
+  int64_t split_len;
+  int64_t tblocks;  // grid.x
+};
+
+constexpr int kThreads = 256;
+constexpr int kTile = 256;      // sources staged in LDS per step
+constexpr int kMaxFields = 5;   // partial fields per target (BARY)
+
+int pass_fields(int kind);
+LaunchCfg choose_launch(int64_t nt, int64_t ns, int precision, int kind,
+                        int num_cu, int force_rpt, int force_splits);
+hipError_t launch_pass(const PassDesc& d, int precision, int kind,
+                       const LaunchCfg& cfg, const Work& w, hipStream_t s);
+hipError_t launch_pass_fp32(const PassDesc& d, int kind, const LaunchCfg& c,
+                            const Work& w, hipStream_t s);
+hipError_t launch_pass_fp64(const PassDesc& d, int kind, const LaunchCfg& c,
+                            const Work& w, hipStream_t s);
+
+// Small device reductions used by the host loops (deterministic, one block).
+// ops[k] selects: 0 sum a, 1 sum |a - b|, 2 sum a*b, 3 max a, 4 sum a log a
+// over a > 0, 5 sum |a|.  Results go to out[k].
+struct ReduceSpec {
+  int nops;
+  int op[8];
+  const double* a[8];
+  const double* b[8];
+  int64_t len[8];
+};
+hipError_t launch_reduce(const ReduceSpec& spec, double* out, hipStream_t s);
+
+// Elementwise helpers.
+hipError_t launch_axpby(double* out, const double* a, double sa, const double* b,
+                        double sb, int64_t n, hipStream_t s);  // out = sa*a + sb*b
+hipError_t launch_log(double* out, const double* a, int64_t n, hipStream_t s);
+hipError_t launch_fill(double* out, double v, int64_t n, hipStream_t s);
+hipError_t launch_etamu(double* out, const double* c, double add, double div,
+                        int64_t n, hipStream_t s);  // out = (c + add) / div
+hipError_t launch_log_minus(double* out, const double* r, const double* a,
+                            int64_t n, hipStream_t s);  // out = log(r) - a
+hipError_t launch_theta_update(double* theta, const double* theta0,
+                               const double* res, const double* c,
+                               const double* etamu, int64_t m, double eta_mu,
+                               int adjust, double minv, double maxv,
+                               double winf, hipStream_t s);
+
+// update_theta_residual (src/solvers/MirrorProx.jl:159-180), overflow-safe form.
+__device__ __forceinline__ double theta_update(double res, double c,
+                                               double etamu, double th0,
+                                               double eta_mu, int adjust,
+                                               double minv, double maxv,
+                                               double winf) {
+  double dif = 2.0 * winf * (res - c) / etamu;
+  double mx = fmax(dif, -dif);
+  double e = exp(dif - mx) * pow((th0 + 1.0) / (1.0 - th0), 1.0 - eta_mu);
+  double em = exp(-mx);
+  double t = (e - em) / (e + em);
+  if (adjust) t = fmin(fmax(t, minv), maxv);
+  return t;
+}
+
+}  // namespace culamp

@@ -1,0 +1,169 @@
+// culamp_pairs.hip -- launch-shape selection and dispatch of the pair-pass
+// kernel (culamp_pair_kernel.h), plus the small reduction and elementwise
+// kernels the host loops use.
+#include "culamp_internal.h"
+
+#include <math.h>
+
+namespace culamp {
+namespace {
+
+constexpr int kSub = 8;  // must match the pair kernel's sub-tile
+
+__global__ __launch_bounds__(1024) void reduce_kernel(const ReduceSpec spec, double* out) {
+  __shared__ double sh[1024];
+  const int tid = threadIdx.x;
+  for (int k = 0; k < spec.nops; ++k) {
+    const int op = spec.op[k];
+    const double* a = spec.a[k];
+    const double* b = spec.b[k];
+    const int64_t n = spec.len[k];
+    double acc = op == 3 ? -INFINITY : 0.0;
+    for (int64_t i = tid; i < n; i += 1024) {
+      const double x = a[i];
+      if (op == 0) acc += x;
+      else if (op == 1) acc += fabs(x - b[i]);
+      else if (op == 2) acc += x * b[i];
+      else if (op == 3) acc = fmax(acc, x);
+      else if (op == 4) { if (x > 0.0) acc += x * log(x); }
+      else acc += fabs(x);
+    }
+    sh[tid] = acc;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+      if (tid < s) sh[tid] = op == 3 ? fmax(sh[tid], sh[tid + s]) : sh[tid] + sh[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) out[k] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void axpby_kernel(double* out, const double* a, double sa,
+                             const double* b, double sb, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = sa * a[i] + sb * b[i];
+}
+__global__ void log_kernel(double* out, const double* a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = log(a[i]);
+}
+__global__ void fill_kernel(double* out, double v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v;
+}
+__global__ void etamu_kernel(double* out, const double* c, double add, double div, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (c[i] + add) / div;
+}
+__global__ void log_minus_kernel(double* out, const double* r, const double* a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = log(r[i]) - a[i];
+}
+__global__ void theta_update_kernel(double* theta, const double* theta0,
+                                    const double* res, const double* c,
+                                    const double* etamu, int64_t m,
+                                    double eta_mu, int adjust, double minv,
+                                    double maxv, double winf) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m)
+    theta[j] = theta_update(res[j], c[j], etamu[j], theta0[j], eta_mu, adjust,
+                            minv, maxv, winf);
+}
+
+inline unsigned lin_blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+
+}  // namespace
+
+int pass_fields(int kind) {
+  return kind == KIND_BARY ? 5 : (kind == KIND_MAX || kind == KIND_MIN) ? 1 : 2;
+}
+
+// Launch shape: the largest targets-per-thread that still gives ~8
+// workgroups per CU once sources are split into chunks of >= 64 points.
+LaunchCfg choose_launch(int64_t nt, int64_t ns, int precision, int kind,
+                        int num_cu, int force_rpt, int force_splits) {
+  LaunchCfg c{};
+  const int64_t want = (int64_t)8 * (num_cu > 0 ? num_cu : 256);
+  const bool fp64_only = kind == KIND_BARY || kind == KIND_MAX || kind == KIND_MIN;
+  const int max_rpt = (precision == 1 && !fp64_only) ? 4 : 2;
+  int rpt = 1;
+  int64_t splits = 1;
+  for (int cand = max_rpt; cand >= 1; cand /= 2) {
+    const int64_t tb = (nt + (int64_t)kThreads * cand - 1) / ((int64_t)kThreads * cand);
+    const int64_t cap = (ns + 63) / 64;
+    int64_t sp = (want + tb - 1) / tb;
+    if (sp > cap) sp = cap;
+    if (sp < 1) sp = 1;
+    rpt = cand;
+    splits = sp;
+    if (tb * sp >= want) break;
+  }
+  if (force_rpt == 1 || force_rpt == 2 || (force_rpt == 4 && max_rpt == 4)) rpt = force_rpt;
+  if (force_splits > 0) splits = force_splits;
+  if (splits > 65535) splits = 65535;
+  int64_t len = (ns + splits - 1) / splits;
+  len = (len + kSub - 1) / kSub * kSub;
+  if (len < kSub) len = kSub;
+  splits = ns > 0 ? (ns + len - 1) / len : 1;
+  c.rpt = rpt;
+  c.splits = (int)splits;
+  c.split_len = len;
+  c.tblocks = (nt + (int64_t)kThreads * rpt - 1) / ((int64_t)kThreads * rpt);
+  if (c.tblocks < 1) c.tblocks = 1;
+  return c;
+}
+
+hipError_t launch_pass(const PassDesc& d, int precision, int kind,
+                       const LaunchCfg& c, const Work& w, hipStream_t s) {
+  const bool fp64_only = kind == KIND_BARY || kind == KIND_MAX || kind == KIND_MIN;
+  if (precision == 1 && !fp64_only) return launch_pass_fp32(d, kind, c, w, s);
+  return launch_pass_fp64(d, kind, c, w, s);
+}
+
+hipError_t launch_reduce(const ReduceSpec& spec, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, s, spec, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpby(double* out, const double* a, double sa, const double* b,
+                        double sb, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(axpby_kernel, dim3(lin_blocks(n)), dim3(256), 0, s, out, a, sa, b, sb, n);
+  return hipGetLastError();
+}
+hipError_t launch_log(double* out, const double* a, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(log_kernel, dim3(lin_blocks(n)), dim3(256), 0, s, out, a, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill(double* out, double v, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel, dim3(lin_blocks(n)), dim3(256), 0, s, out, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_etamu(double* out, const double* c, double add, double div,
+                        int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(etamu_kernel, dim3(lin_blocks(n)), dim3(256), 0, s, out, c, add, div, n);
+  return hipGetLastError();
+}
+hipError_t launch_log_minus(double* out, const double* r, const double* a,
+                            int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(log_minus_kernel, dim3(lin_blocks(n)), dim3(256), 0, s, out, r, a, n);
+  return hipGetLastError();
+}
+hipError_t launch_theta_update(double* theta, const double* theta0,
+                               const double* res, const double* c,
+                               const double* etamu, int64_t m, double eta_mu,
+                               int adjust, double minv, double maxv,
+                               double winf, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(theta_update_kernel, dim3(lin_blocks(m)), dim3(256), 0, s,
+                     theta, theta0, res, c, etamu, m, eta_mu, adjust, minv, maxv, winf);
+  return hipGetLastError();
+}
+
+}  // namespace culamp

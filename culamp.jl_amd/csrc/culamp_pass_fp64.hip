@@ -1,0 +1,15 @@
+// culamp_pass_fp64.hip -- FP64 instantiations of the pair-pass kernel.
+#include "culamp_pair_kernel.h"
+
+namespace culamp {
+hipError_t launch_pass_fp64(const PassDesc& d, int kind, const LaunchCfg& c,
+                            const Work& w, hipStream_t s) {
+  switch (kind) {
+    case KIND_LSE: return launch_cost<0, KIND_LSE>(d, c, w, s);
+    case KIND_SUM: return launch_cost<0, KIND_SUM>(d, c, w, s);
+    case KIND_BARY: return launch_cost<0, KIND_BARY>(d, c, w, s);
+    case KIND_MAX: return launch_cost<0, KIND_MAX>(d, c, w, s);
+    default: return launch_cost<0, KIND_MIN>(d, c, w, s);
+  }
+}
+}  // namespace culamp

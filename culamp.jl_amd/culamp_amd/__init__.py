@@ -1,0 +1,24 @@
+"""culamp_amd -- host-side mirror of CuLAMP.jl's on-the-fly solvers over the
+libculamp C ABI (MI355X / gfx950 HIP kernels).
+
+Solver entry points keep the reference's names, argument order and return
+values (src/solvers/MirrorProxKernel.jl, SinkhornKernel.jl,
+AnnealedSinkhorn.jl); kernel-level functions keep the reference kernel names
+(without the trailing ``!``).  Point clouds are ``(n, 3)`` C-contiguous float64
+arrays, i.e. the same bytes as the reference's ``3 x n`` column-major
+matrices.
+"""
+from ._lib import (FP64, MIXED, CulampError, Context, DeviceArray, device_count,
+                   load)
+from .args import EOTArgs, args_from_dict, read_args_json
+from .solvers import (annealed_sinkhorn_euclidean, default_context,
+                      extragradient_color_transfer, extragradient_euclidean,
+                      sinkhorn_color_transfer, sinkhorn_euclidean, Problem)
+from . import kernels
+
+__all__ = [
+    "FP64", "MIXED", "CulampError", "Context", "DeviceArray", "device_count", "load",
+    "EOTArgs", "args_from_dict", "read_args_json", "annealed_sinkhorn_euclidean",
+    "default_context", "extragradient_color_transfer", "extragradient_euclidean",
+    "sinkhorn_color_transfer", "sinkhorn_euclidean", "Problem", "kernels",
+]

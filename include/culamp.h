@@ -1,0 +1,225 @@
+/*
+ * culamp.h -- C ABI of libculamp, the MI355X (gfx950) implementation of
+ * CuLAMP.jl's fused on-the-fly cost + row/column log-sum-exp hot path.
+ *
+ * Every entry point returns an int status (CULAMP_OK == 0).  On failure
+ * culamp_last_error() returns a thread-local message.  No entry point retains a
+ * caller pointer after it returns.
+ *
+ * Two levels mirror the reference:
+ *   - kernel level: one symbol per device kernel of the reference, same math
+ *     and argument meaning, operating on DEVICE pointers (allocated with
+ *     culamp_malloc or any HIP allocation on the context's device) on the
+ *     context's stream.  Calls are stream-ordered; culamp_ctx_synchronize
+ *     waits for them.
+ *   - solver level: one symbol per reference solver, taking HOST arrays (or a
+ *     device-resident culamp_problem) and writing HOST outputs.  Solver calls
+ *     return when their results are in the caller's buffers.
+ *
+ * Point clouds are 3 x n column-major FP64 (x[3*i + k]) as the reference's
+ * `img1::CuDeviceMatrix{Float64}` / `locations` arrays.  The cost is the
+ * reference kernels' `l2dist`: p == 1 sum|d|, p == 2 SQUARED Euclidean,
+ * p == +Inf max|d|, otherwise sum|d|^p.
+ *
+ * precision: CULAMP_FP64 runs every pair in IEEE double.  CULAMP_MIXED runs
+ * the cost and the exp-sum per pair in FP32 while potentials, partial-sum
+ * merges, marginals and infeasibility stay FP64.
+ *
+ * One context owns one HIP stream; calls on one context are not reentrant.
+ * Different contexts may be used from different threads.
+ */
+#ifndef CULAMP_H
+#define CULAMP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CULAMP_ABI_VERSION 1
+
+enum {
+  CULAMP_OK = 0,
+  CULAMP_ERR_INVALID_ARG = 1,
+  CULAMP_ERR_HIP = 2,
+  CULAMP_ERR_NONFINITE = 3,
+  CULAMP_ERR_NOT_CONVERGED = 4,
+  CULAMP_ERR_NO_DEVICE = 5,
+  CULAMP_ERR_ALLOC = 6
+};
+
+enum { CULAMP_FP64 = 0, CULAMP_MIXED = 1 };
+enum { CULAMP_STOP_REFERENCE = 0, CULAMP_STOP_INFEAS = 1 };
+
+typedef struct culamp_ctx culamp_ctx;
+typedef struct culamp_problem culamp_problem;
+
+/* Mirrors EOTArgs (src/Utilities.jl:18-31) field for field, same defaults via
+ * culamp_args_default.  Two fields are new and default to the reference's
+ * behaviour: `precision` (CULAMP_FP64) and `stop` (CULAMP_STOP_REFERENCE).
+ * CULAMP_STOP_INFEAS stops LAMP when the L1 infeasibility is below
+ * epsilon/6, tested every `frequency` iterations whether or not `verbose` is
+ * set (the reference tests primal - dual and infeasibility, and only when
+ * verbose, MirrorProxKernel.jl:1081,1098). */
+typedef struct {
+  double eta_p, eta_mu, tau_p, tau_mu, alpha, B, epsilon, anneal_mult;
+  int64_t itermax, inner_iter;
+  double tmax;
+  int32_t verbose;
+  int32_t precision;
+  int32_t stop;
+  int32_t reserved;
+} culamp_args;
+
+/* Solver summary (fields the reference returns or logs). */
+typedef struct {
+  int64_t iterations;  /* completed iterations (Sinkhorn: num_iter) */
+  double objective;    /* transport cost <P, C> at exit */
+  double residual;     /* L1 marginal infeasibility at exit */
+  double primal, dual; /* last logged primal / dual values (NaN if none) */
+  double winf;         /* W_inf = max_ij C_ij */
+  double eta_t;        /* LAMP: final eta_t; Sinkhorn: eta */
+  double elapsed_s;    /* wall time of the solve */
+  double pairs;        /* cost + exp evaluations issued (sum of n*m over
+                          O(n*m) passes), for throughput accounting */
+  int64_t passes;      /* O(n*m) passes issued */
+  int32_t converged;   /* stop test met */
+  int32_t nonfinite;   /* a NaN/Inf was seen in the residual or objective */
+} culamp_result;
+
+/* Log sink: called synchronously on the calling thread with one CSV line in
+ * the reference's exact format (header and rows, trailing newline). */
+typedef void (*culamp_log_fn)(void *user, const char *line);
+
+/* ---- library / context -------------------------------------------------- */
+int culamp_abi_version(void);
+const char *culamp_last_error(void);
+void culamp_args_default(culamp_args *args);
+int culamp_device_count(int *count);
+int culamp_ctx_create(int device, culamp_ctx **out);
+int culamp_ctx_destroy(culamp_ctx *ctx);
+int culamp_ctx_synchronize(culamp_ctx *ctx);
+/* Launch-shape override (no reference counterpart; used by tests and the
+ * tuner): targets per thread in {1, 2, 4} and number of source splits; 0
+ * selects automatically. */
+int culamp_ctx_set_launch(culamp_ctx *ctx, int rows_per_thread, int splits);
+/* Device memory helpers (stream-ordered on the context's stream). */
+int culamp_malloc(culamp_ctx *ctx, size_t bytes, void **dptr);
+int culamp_free(culamp_ctx *ctx, void *dptr);
+int culamp_memcpy_htod(culamp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int culamp_memcpy_dtoh(culamp_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* HIP-event timer on the context's stream: start, then stop returns the
+ * elapsed milliseconds between the two events (waits for the stop event). */
+int culamp_timer_start(culamp_ctx *ctx);
+int culamp_timer_stop(culamp_ctx *ctx, double *ms);
+
+/* ---- kernel level (device pointers) -------------------------------------- */
+/* max_logsumexp_spp_ct! (src/solvers/MirrorProxKernel.jl:280-326):
+ * out[i] = max_j C(x_i, y_j). */
+int culamp_max_cost(culamp_ctx *ctx, const double *x, int64_t n,
+                    const double *y, int64_t m, double p, double *out);
+/* warp_logsumexp_spp_ct_fused! (MirrorProxKernel.jl:833-938), same math as
+ * warp_logsumexp_spp_ct_opt! (702-829):
+ * out[i] = log sum_j exp(-(C(x_i,y_j) * st/(2 winf) + theta[j]) / reg). */
+int culamp_lse_rows(culamp_ctx *ctx, const double *x, int64_t n,
+                    const double *y, int64_t m, const double *theta,
+                    double reg, double st, double winf, double p,
+                    int precision, double *out);
+/* residual_spp_c! (MirrorProxKernel.jl:7-94): for each column j
+ *   w_ij = r[i] exp(-(C_ij st/(2 winf) + theta[j]) / reg - logZ[i])
+ *   res[j] = sum_i w_ij,  cost[j] = sum_i w_ij C_ij   (cost may be NULL). */
+int culamp_col_marginal_cost(culamp_ctx *ctx, const double *x, int64_t n,
+                             const double *y, int64_t m, const double *r,
+                             const double *theta, const double *logZ,
+                             double reg, double st, double winf, double p,
+                             int precision, double *res, double *cost);
+/* warp_min_reduce! (MirrorProxKernel.jl:941-1016):
+ * out[i] = min_j C(x_i,y_j) + 2 winf theta[j]. */
+int culamp_minplus_rows(culamp_ctx *ctx, const double *x, int64_t n,
+                        const double *y, int64_t m, const double *theta,
+                        double winf, double p, double *out);
+/* update_theta_residual (src/solvers/MirrorProx.jl:159-180), length m:
+ * theta = tanh((res-c) winf/eta_mu_i... ) in the reference's overflow-safe
+ * form; clamped to [minv, maxv] when adjust != 0.  theta may alias theta0. */
+int culamp_lamp_theta_update(culamp_ctx *ctx, double *theta,
+                             const double *theta0, const double *res,
+                             const double *c, const double *eta_mu_i,
+                             int64_t m, double eta_mu, int adjust, double minv,
+                             double maxv, double winf);
+/* warp_logsumexp_ct_fused! (src/solvers/SinkhornKernel.jl:341-444):
+ * out[i] = log(marg[i]) - log sum_j exp(-C(x_i,y_j)/(reg winf) + psi[j]).
+ * The reference's p == Inf Sinkhorn uses warp_logsumexp_ct_opt! (230-338),
+ * which evaluates squared L2: pass p = 2 for that behaviour. */
+int culamp_sinkhorn_half_step(culamp_ctx *ctx, const double *x, int64_t n,
+                              const double *y, int64_t m, const double *marg,
+                              const double *psi, double reg, double winf,
+                              double p, int precision, double *out);
+/* residual_opt! (SinkhornKernel.jl:9-81):
+ *   w_ij = exp(-C_ij/(reg winf) + phi[i] + psi[j])
+ *   res[i] = marg[i] - sum_j w_ij,  cost[i] = sum_j w_ij C_ij (may be NULL). */
+int culamp_row_residual_cost(culamp_ctx *ctx, const double *x, int64_t n,
+                             const double *y, int64_t m, const double *marg,
+                             const double *phi, const double *psi, double reg,
+                             double winf, double p, int precision, double *res,
+                             double *cost);
+/* Barycentric projections, out is 3 x n:
+ * naive_findmaxindex_ct! (SinkhornKernel.jl:124-162):
+ *   out_i = sum_j y_j w_ij / sum_j w_ij, w_ij = exp(-C/(reg winf) + phi_i + psi_j) */
+int culamp_barycentric_sinkhorn(culamp_ctx *ctx, const double *x, int64_t n,
+                                const double *y, int64_t m, const double *phi,
+                                const double *psi, double reg, double winf,
+                                double p, double *out);
+/* naive_findmaxindex_spp_ct! (MirrorProxKernel.jl:97-142), rows (out: 3 x n):
+ *   w_ij = r_i exp(-(C/(2 winf) + nu_j)/reg - logZ_i) */
+int culamp_barycentric_lamp_rows(culamp_ctx *ctx, const double *x, int64_t n,
+                                 const double *y, int64_t m, const double *nu,
+                                 const double *r, const double *logZ,
+                                 double reg, double winf, double p,
+                                 double *out);
+/* naive_findmaxindex_spp_ct_t! (MirrorProxKernel.jl:144-189), columns
+ * (out: 3 x m): mean of x_i under w_ij for fixed j. */
+int culamp_barycentric_lamp_cols(culamp_ctx *ctx, const double *x, int64_t n,
+                                 const double *y, int64_t m, const double *r,
+                                 const double *nu, const double *logZ,
+                                 double reg, double winf, double p,
+                                 double *out);
+
+/* ---- problems (device-resident inputs) ----------------------------------- */
+/* Copies x (3 x n), y (3 x m), r (n), c (m) from host memory to the device. */
+int culamp_problem_create(culamp_ctx *ctx, const double *x, int64_t n,
+                          const double *y, int64_t m, const double *r,
+                          const double *c, culamp_problem **out);
+int culamp_problem_destroy(culamp_problem *prob);
+
+/* ---- solver level --------------------------------------------------------- */
+/* extragradient_color_transfer (src/solvers/MirrorProxKernel.jl:1019-1132):
+ * LAMP mirror prox with on-the-fly cost.  Outputs (any may be NULL):
+ *   nu (m), phi (n) = log r - logZ, psi (m) = -2 winf nu / eta_p,
+ *   img1 (3 x n) and img2 (3 x m) barycentric projections. */
+int culamp_lamp_solve(culamp_ctx *ctx, culamp_problem *prob,
+                      const culamp_args *args, int64_t frequency, double p,
+                      culamp_log_fn log_fn, void *user, double *nu, double *phi,
+                      double *psi, double *img1, double *img2,
+                      culamp_result *result);
+/* sinkhorn_color_transfer (src/solvers/SinkhornKernel.jl:477-559).  phi0 (n)
+ * and psi0 (m) are optional warm starts (NULL = zeros).  img1/img2 may be
+ * NULL (return_assignments = false). */
+int culamp_sinkhorn_solve(culamp_ctx *ctx, culamp_problem *prob,
+                          const culamp_args *args, int64_t frequency, double p,
+                          const double *phi0, const double *psi0,
+                          culamp_log_fn log_fn, void *user, double *phi,
+                          double *psi, double *img1, double *img2,
+                          culamp_result *result);
+/* annealed_sinkhorn_euclidean (src/solvers/AnnealedSinkhorn.jl:65-177). */
+int culamp_annealed_sinkhorn_solve(culamp_ctx *ctx, culamp_problem *prob,
+                                   const culamp_args *args, int64_t frequency,
+                                   double p, culamp_log_fn log_fn, void *user,
+                                   double *phi, double *psi, double *img1,
+                                   double *img2, culamp_result *result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,263 @@
+"""ctypes loader for the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product library never
+loads it.
+
+Arrays follow the oracle's C conventions: points (n, 3) C-contiguous float64
+(= 3 x n column-major), vectors float64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class or_args(C.Structure):
+    _fields_ = [
+        ("eta_p", C.c_double), ("eta_mu", C.c_double), ("tau_p", C.c_double),
+        ("tau_mu", C.c_double), ("alpha", C.c_double), ("B", C.c_double),
+        ("epsilon", C.c_double), ("anneal_mult", C.c_double),
+        ("itermax", C.c_int64), ("inner_iter", C.c_int64), ("tmax", C.c_double),
+        ("verbose", C.c_int32), ("stop", C.c_int32),
+    ]
+
+
+class or_result(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64), ("objective", C.c_double), ("residual", C.c_double),
+        ("primal", C.c_double), ("dual", C.c_double), ("winf", C.c_double),
+        ("eta_t", C.c_double), ("converged", C.c_int32), ("_pad", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "_pad"}
+
+
+LOG_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)
+_P, _D, _I64, _I = C.c_void_p, C.c_double, C.c_int64, C.c_int
+
+_SIG = {
+    "or_set_threads": (None, [_I]),
+    "or_get_threads": (_I, []),
+    "or_cost3": (_D, [_P, _P, _D]),
+    "or_max_cost_rows": (None, [_P, _I64, _P, _I64, _D, _P]),
+    "or_lse_rows": (None, [_P, _I64, _P, _I64, _P, _D, _D, _D, _D, _P]),
+    "or_col_marginal_cost": (None, [_P, _I64, _P, _I64, _P, _P, _P, _D, _D, _D, _D, _P, _P]),
+    "or_minplus_rows": (None, [_P, _I64, _P, _I64, _P, _D, _D, _P]),
+    "or_theta_update": (None, [_P, _P, _P, _P, _P, _I64, _D, _I, _D, _D, _D]),
+    "or_sinkhorn_half_step": (None, [_P, _I64, _P, _I64, _P, _P, _D, _D, _D, _P]),
+    "or_row_residual_cost": (None, [_P, _I64, _P, _I64, _P, _P, _P, _D, _D, _D, _P, _P]),
+    "or_barycentric_sinkhorn": (None, [_P, _I64, _P, _I64, _P, _P, _D, _D, _D, _P]),
+    "or_barycentric_lamp_rows": (None, [_P, _I64, _P, _I64, _P, _P, _P, _D, _D, _D, _D, _P]),
+    "or_barycentric_lamp_cols": (None, [_P, _I64, _P, _I64, _P, _P, _P, _D, _D, _D, _D, _P]),
+    "or_lamp_solve": (_I, [_P, _I64, _P, _I64, _P, _P, C.POINTER(or_args), _I64, _D, LOG_FN, _P,
+                           _P, _P, _P, _P, _P, C.POINTER(or_result)]),
+    "or_sinkhorn_solve": (_I, [_P, _I64, _P, _I64, _P, _P, C.POINTER(or_args), _I64, _D, _P, _P,
+                               LOG_FN, _P, _P, _P, _P, _P, C.POINTER(or_result)]),
+    "or_annealed_sinkhorn_solve": (_I, [_P, _I64, _P, _I64, _P, _P, C.POINTER(or_args), _I64, _D,
+                                        LOG_FN, _P, _P, _P, _P, _P, C.POINTER(or_result)]),
+    "or_sinkhorn_log_dense": (_I, [_P, _I64, _P, _I64, _P, C.POINTER(or_args), _I64, LOG_FN, _P,
+                                   _P, _P, _P, C.POINTER(or_result)]),
+    "or_grid_cost_matrix": (None, [_I64, _I64, _D, _P]),
+}
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        for k, (res, args) in _SIG.items():
+            f = getattr(L, k)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def set_threads(n: int) -> None:
+    lib().or_set_threads(int(n))
+
+
+def make_args(eta_p=0.0, eta_mu=0.0, tau_p=1.0, tau_mu=1.0, alpha=0.01, B=1.0, epsilon=1e-4,
+              anneal_mult=0.95, itermax=10000, inner_iter=10, tmax=float("inf"), verbose=True,
+              stop="reference", **_ignored) -> or_args:
+    a = or_args()
+    a.eta_p, a.eta_mu, a.tau_p, a.tau_mu = eta_p, eta_mu, tau_p, tau_mu
+    a.alpha, a.B, a.epsilon, a.anneal_mult = alpha, B, epsilon, anneal_mult
+    a.itermax, a.inner_iter, a.tmax, a.verbose = int(itermax), int(inner_iter), tmax, int(bool(verbose))
+    a.stop = {"reference": 0, "infeas": 1}[stop]
+    return a
+
+
+def cost3(a, b, p) -> float:
+    a, b = _f(a), _f(b)
+    return lib().or_cost3(_p(a), _p(b), float(p))
+
+
+def max_cost_rows(x, y, p):
+    x, y = _f(x), _f(y)
+    out = np.empty(x.shape[0])
+    lib().or_max_cost_rows(_p(x), x.shape[0], _p(y), y.shape[0], float(p), _p(out))
+    return out
+
+
+def lse_rows(x, y, theta, reg, st, winf, p):
+    x, y, theta = _f(x), _f(y), _f(theta)
+    out = np.empty(x.shape[0])
+    lib().or_lse_rows(_p(x), x.shape[0], _p(y), y.shape[0], _p(theta), reg, st, winf, p, _p(out))
+    return out
+
+
+def col_marginal_cost(x, y, r, theta, logZ, reg, st, winf, p):
+    x, y, r, theta, logZ = map(_f, (x, y, r, theta, logZ))
+    res = np.empty(y.shape[0])
+    cost = np.empty(y.shape[0])
+    lib().or_col_marginal_cost(_p(x), x.shape[0], _p(y), y.shape[0], _p(r), _p(theta), _p(logZ),
+                               reg, st, winf, p, _p(res), _p(cost))
+    return res, cost
+
+
+def minplus_rows(x, y, theta, winf, p):
+    x, y, theta = map(_f, (x, y, theta))
+    out = np.empty(x.shape[0])
+    lib().or_minplus_rows(_p(x), x.shape[0], _p(y), y.shape[0], _p(theta), winf, p, _p(out))
+    return out
+
+
+def theta_update(theta0, res, c, eta_mu_i, eta_mu, adjust, minv, maxv, winf):
+    theta0, res, c, eta_mu_i = map(_f, (theta0, res, c, eta_mu_i))
+    out = np.empty(theta0.size)
+    lib().or_theta_update(_p(out), _p(theta0), _p(res), _p(c), _p(eta_mu_i), theta0.size, eta_mu,
+                          int(adjust), minv, maxv, winf)
+    return out
+
+
+def sinkhorn_half_step(x, y, marg, psi, reg, winf, p):
+    x, y, marg, psi = map(_f, (x, y, marg, psi))
+    out = np.empty(x.shape[0])
+    lib().or_sinkhorn_half_step(_p(x), x.shape[0], _p(y), y.shape[0], _p(marg), _p(psi), reg, winf,
+                                p, _p(out))
+    return out
+
+
+def row_residual_cost(x, y, marg, phi, psi, reg, winf, p):
+    x, y, marg, phi, psi = map(_f, (x, y, marg, phi, psi))
+    res = np.empty(x.shape[0])
+    cost = np.empty(x.shape[0])
+    lib().or_row_residual_cost(_p(x), x.shape[0], _p(y), y.shape[0], _p(marg), _p(phi), _p(psi),
+                               reg, winf, p, _p(res), _p(cost))
+    return res, cost
+
+
+def barycentric_sinkhorn(x, y, phi, psi, reg, winf, p):
+    x, y, phi, psi = map(_f, (x, y, phi, psi))
+    out = np.empty((x.shape[0], 3))
+    lib().or_barycentric_sinkhorn(_p(x), x.shape[0], _p(y), y.shape[0], _p(phi), _p(psi), reg,
+                                  winf, p, _p(out))
+    return out
+
+
+def barycentric_lamp_rows(x, y, nu, r, logZ, reg, st, winf, p):
+    x, y, nu, r, logZ = map(_f, (x, y, nu, r, logZ))
+    out = np.empty((x.shape[0], 3))
+    lib().or_barycentric_lamp_rows(_p(x), x.shape[0], _p(y), y.shape[0], _p(nu), _p(r), _p(logZ),
+                                   reg, st, winf, p, _p(out))
+    return out
+
+
+def barycentric_lamp_cols(x, y, r, nu, logZ, reg, st, winf, p):
+    x, y, r, nu, logZ = map(_f, (x, y, r, nu, logZ))
+    out = np.empty((y.shape[0], 3))
+    lib().or_barycentric_lamp_cols(_p(x), x.shape[0], _p(y), y.shape[0], _p(r), _p(nu), _p(logZ),
+                                   reg, st, winf, p, _p(out))
+    return out
+
+
+class _Log:
+    def __init__(self):
+        self.lines: list[str] = []
+        self.cb = LOG_FN(lambda _u, s: self.lines.append(s.decode()))
+
+
+def lamp_solve(x, y, r, c, frequency=100, p=2.0, **args):
+    x, y, r, c = map(_f, (x, y, r, c))
+    n, m = x.shape[0], y.shape[0]
+    nu, phi, psi = np.empty(m), np.empty(n), np.empty(m)
+    img1, img2 = np.empty((n, 3)), np.empty((m, 3))
+    a, res, lg = make_args(**args), or_result(), _Log()
+    rc = lib().or_lamp_solve(_p(x), n, _p(y), m, _p(r), _p(c), C.byref(a), int(frequency),
+                             float(p), lg.cb, None, _p(nu), _p(phi), _p(psi), _p(img1), _p(img2),
+                             C.byref(res))
+    assert rc == 0
+    return dict(nu=nu, phi=phi, psi=psi, img1=img1, img2=img2, log=lg.lines, **res.as_dict())
+
+
+def sinkhorn_solve(x, y, r, c, frequency=100, p=2.0, phi0=None, psi0=None, **args):
+    x, y, r, c = map(_f, (x, y, r, c))
+    n, m = x.shape[0], y.shape[0]
+    phi, psi = np.empty(n), np.empty(m)
+    img1, img2 = np.empty((n, 3)), np.empty((m, 3))
+    p0 = None if phi0 is None else _f(phi0)
+    q0 = None if psi0 is None else _f(psi0)
+    a, res, lg = make_args(**args), or_result(), _Log()
+    rc = lib().or_sinkhorn_solve(_p(x), n, _p(y), m, _p(r), _p(c), C.byref(a), int(frequency),
+                                 float(p), _p(p0), _p(q0), lg.cb, None, _p(phi), _p(psi),
+                                 _p(img1), _p(img2), C.byref(res))
+    assert rc == 0
+    return dict(phi=phi, psi=psi, img1=img1, img2=img2, log=lg.lines, **res.as_dict())
+
+
+def annealed_sinkhorn_solve(x, y, r, c, frequency=100, p=2.0, **args):
+    x, y, r, c = map(_f, (x, y, r, c))
+    n, m = x.shape[0], y.shape[0]
+    phi, psi = np.empty(n), np.empty(m)
+    img1, img2 = np.empty((n, 3)), np.empty((m, 3))
+    a, res, lg = make_args(**args), or_result(), _Log()
+    rc = lib().or_annealed_sinkhorn_solve(_p(x), n, _p(y), m, _p(r), _p(c), C.byref(a),
+                                          int(frequency), float(p), lg.cb, None, _p(phi), _p(psi),
+                                          _p(img1), _p(img2), C.byref(res))
+    assert rc == 0
+    return dict(phi=phi, psi=psi, img1=img1, img2=img2, log=lg.lines, **res.as_dict())
+
+
+def sinkhorn_log_dense(r, c, W, frequency=50, **args):
+    r, c, W = map(_f, (r, c, W))
+    n, m = r.size, c.size
+    phi, psi, P = np.empty(n), np.empty(m), np.empty((n, m))
+    a, res, lg = make_args(**args), or_result(), _Log()
+    rc = lib().or_sinkhorn_log_dense(_p(r), n, _p(c), m, _p(W), C.byref(a), int(frequency), lg.cb,
+                                     None, _p(phi), _p(psi), _p(P), C.byref(res))
+    assert rc == 0
+    return dict(phi=phi, psi=psi, P=P, log=lg.lines, **res.as_dict())
+
+
+def grid_cost_matrix(h, w, p):
+    W = np.empty((h * w, h * w))
+    lib().or_grid_cost_matrix(h, w, float(p), _p(W))
+    return W

@@ -1,0 +1,234 @@
+"""Kernel-level parity: every libculamp kernel entry point against the golden
+vectors and the FP64 oracle, FP64 and mixed precision, ragged sizes, forced
+launch shapes (targets per thread x source splits) and edge cases."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import pval, unhex
+
+pytestmark = pytest.mark.gpu
+
+# Mixed precision tolerance on an LSE output: FP32 cost/exponent rounding is
+# relative 2^-24 per term; the bound is in units of the largest |exponent|
+# of the row (4 ulps) plus an FP64-merge floor.
+MIXED_LSE_ULPS = 4 * 2.0 ** -24
+# Mixed precision tolerance on sums (marginals, costs): relative to the sum.
+MIXED_SUM_REL = 2e-5
+FP64_REL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(ctx, a):
+    import culamp_amd
+    return culamp_amd.DeviceArray.from_host(ctx, np.ascontiguousarray(a, dtype=np.float64))
+
+
+def out(ctx, n):
+    import culamp_amd
+    return culamp_amd.DeviceArray(ctx, n)
+
+
+def rel(a, b, floor=1e-300):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+def test_golden_lamp(gpu_ctx, golden):
+    from culamp_amd import kernels as K
+    for c in golden:
+        if c["kind"] != "lamp":
+            continue
+        say("golden lamp p", c["p"], "reg", c["reg"])
+        x, y = unhex(c["x"], (-1, 3)), unhex(c["y"], (-1, 3))
+        n, m = len(x), len(y)
+        p, reg = pval(c["p"]), pval(c["reg"])
+        dx, dy, th, r = dev(gpu_ctx, x), dev(gpu_ctx, y), dev(gpu_ctx, unhex(c["theta"])), dev(gpu_ctx, unhex(c["r"]))
+        o = out(gpu_ctx, n)
+        K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, dy, th, reg, c["st"], c["winf"], p)
+        assert rel(o.get(), c["logZ"]) < FP64_REL
+        lz = dev(gpu_ctx, unhex(c["logZ_in"]))
+        res, cost = out(gpu_ctx, m), out(gpu_ctx, m)
+        K.residual_spp_c(gpu_ctx, res, cost, dx, dy, r, th, lz, reg, c["st"], c["winf"], p)
+        assert rel(res.get(), c["res"]) < FP64_REL
+        assert rel(cost.get(), c["cost"]) < FP64_REL
+        K.warp_min_reduce(gpu_ctx, o, dx, dy, th, c["winf"], p)
+        assert rel(o.get(), c["minplus"]) < FP64_REL
+        K.max_logsumexp_spp_ct(gpu_ctx, o, dx, dy, p)
+        assert rel(o.get(), c["maxcost"]) < 1e-14
+        if "bary_rows" in c:
+            b1, b2 = out(gpu_ctx, 3 * n), out(gpu_ctx, 3 * m)
+            K.naive_findmaxindex_spp_ct(gpu_ctx, b1, dx, dy, th, r, lz, reg, 1.0, c["winf"], p)
+            K.naive_findmaxindex_spp_ct_t(gpu_ctx, b2, dx, dy, r, th, lz, reg, 1.0, c["winf"], p)
+            assert rel(b1.get().reshape(-1, 3), c["bary_rows"]) < 1e-11
+            assert rel(b2.get().reshape(-1, 3), c["bary_cols"]) < 1e-11
+
+
+def test_golden_sinkhorn(gpu_ctx, golden):
+    from culamp_amd import kernels as K
+    for c in golden:
+        if c["kind"] != "sinkhorn":
+            continue
+        say("golden sinkhorn p", c["p"], "reg", c["reg"])
+        x, y = unhex(c["x"], (-1, 3)), unhex(c["y"], (-1, 3))
+        n = len(x)
+        p = pval(c["p"])
+        dx, dy = dev(gpu_ctx, x), dev(gpu_ctx, y)
+        mg, phi, psi = dev(gpu_ctx, unhex(c["marg"])), dev(gpu_ctx, unhex(c["phi"])), dev(gpu_ctx, unhex(c["psi"]))
+        o = out(gpu_ctx, n)
+        K.warp_logsumexp_ct_fused(gpu_ctx, o, dx, dy, mg, psi, c["reg"], c["winf"], p)
+        assert rel(o.get(), c["half"]) < FP64_REL
+        if p in (1.0, 2.0):  # residual_opt / naive_findmaxindex_ct know p = 1 or squared L2
+            res, cost = out(gpu_ctx, n), out(gpu_ctx, n)
+            K.residual_opt(gpu_ctx, res, cost, dx, dy, mg, phi, psi, c["reg"], c["winf"], p)
+            scale = max(1.0, float(np.max(np.abs(c["resid"]))))
+            assert np.max(np.abs(res.get() - np.array(c["resid"]))) < 1e-12 * scale
+            assert rel(cost.get(), c["rcost"]) < FP64_REL
+            b = out(gpu_ctx, 3 * n)
+            K.naive_findmaxindex_ct(gpu_ctx, b, dx, dy, phi, psi, c["reg"], c["winf"], p)
+            assert rel(b.get().reshape(-1, 3), c["bary"]) < 1e-11
+
+
+def test_golden_theta(gpu_ctx, golden):
+    from culamp_amd import kernels as K
+    for c in golden:
+        if c["kind"] != "theta":
+            continue
+        m = len(c["out"])
+        th = out(gpu_ctx, m)
+        K.update_theta_residual(gpu_ctx, th, dev(gpu_ctx, unhex(c["theta0"])), dev(gpu_ctx, unhex(c["res"])),
+                                dev(gpu_ctx, unhex(c["c"])), dev(gpu_ctx, unhex(c["etamu"])), c["eta_mu"],
+                                bool(c["adjust"]), c["minv"], c["maxv"], 1.0)
+        assert rel(th.get(), c["out"]) < 1e-13
+
+
+def _inputs(seed, n, m):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(size=(n, 3))
+    y = rng.uniform(size=(m, 3))
+    th = rng.uniform(-1, 1, m)
+    r = rng.uniform(0.1, 1, n)
+    r /= r.sum()
+    return x, y, th, r
+
+
+SHAPES = [(1, 1), (3, 5), (7, 300), (257, 129), (1000, 777), (2048, 4096)]
+LAUNCH = [(0, 0), (1, 1), (1, 3), (2, 17), (4, 5)]
+
+
+@pytest.mark.parametrize("n,m", SHAPES)
+@pytest.mark.parametrize("p", [1.0, 2.0, math.inf, 3.0])
+def test_lse_and_colmarg_vs_oracle(gpu_ctx, oracle, n, m, p):
+    from culamp_amd import FP64, MIXED
+    from culamp_amd import kernels as K
+    say("lse/colmarg n", n, "m", m, "p", p)
+    x, y, th, r = _inputs(n * 7 + m, n, m)
+    winf = float(oracle.max_cost_rows(x, y, p).max())
+    reg, st = 0.02, 1.0
+    ref = oracle.lse_rows(x, y, th, reg, st, winf, p)
+    res_ref, cost_ref = oracle.col_marginal_cost(x, y, r, th, ref, reg, st, winf, p)
+    dx, dy, dth, dr = dev(gpu_ctx, x), dev(gpu_ctx, y), dev(gpu_ctx, th), dev(gpu_ctx, r)
+    dlz = dev(gpu_ctx, ref)
+    o, res, cost = out(gpu_ctx, n), out(gpu_ctx, m), out(gpu_ctx, m)
+    # exponent magnitude bound per row for the mixed tolerance
+    vmax = (np.max(oracle.max_cost_rows(x, y, p)) / (2 * winf) + 1.0) / reg
+    for rpt, splits in LAUNCH:
+        gpu_ctx.set_launch(rpt, splits)
+        try:
+            K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, dy, dth, reg, st, winf, p, FP64)
+            assert rel(o.get(), ref) < FP64_REL, (rpt, splits)
+            K.residual_spp_c(gpu_ctx, res, cost, dx, dy, dr, dth, dlz, reg, st, winf, p, FP64)
+            assert rel(res.get(), res_ref, 1e-300) < 1e-11, (rpt, splits)
+            assert rel(cost.get(), cost_ref, 1e-300) < 1e-11, (rpt, splits)
+            K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, dy, dth, reg, st, winf, p, MIXED)
+            err = np.max(np.abs(o.get() - ref))
+            assert err < MIXED_LSE_ULPS * vmax + 1e-9, (rpt, splits, err)
+            K.residual_spp_c(gpu_ctx, res, cost, dx, dy, dr, dth, dlz, reg, st, winf, p, MIXED)
+            # marginal sums: relative to the total mass, scaled by the exponent size
+            tol = MIXED_SUM_REL * max(1.0, vmax * 2.0 ** -24 * 1e4)
+            assert np.max(np.abs(res.get() - res_ref)) <= tol * max(res_ref.max(), 1e-300), (rpt, splits)
+        finally:
+            gpu_ctx.set_launch(0, 0)
+
+
+def test_deterministic_repeat(gpu_ctx, oracle):
+    from culamp_amd import FP64, MIXED
+    from culamp_amd import kernels as K
+    x, y, th, r = _inputs(11, 3000, 5000)
+    dx, dy, dth = dev(gpu_ctx, x), dev(gpu_ctx, y), dev(gpu_ctx, th)
+    o = out(gpu_ctx, 3000)
+    for prec in (FP64, MIXED):
+        K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, dy, dth, 0.01, 1.0, 3.0, 2.0, prec)
+        a = o.get()
+        for _ in range(3):
+            K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, dy, dth, 0.01, 1.0, 3.0, 2.0, prec)
+            assert np.array_equal(a, o.get())
+
+
+@pytest.mark.parametrize("n,m", [(1, 1), (5, 3), (700, 1100), (4096, 4096)])
+@pytest.mark.parametrize("p", [1.0, 2.0, math.inf, 3.0])
+def test_sinkhorn_kernels_vs_oracle(gpu_ctx, oracle, n, m, p):
+    from culamp_amd import FP64, MIXED
+    from culamp_amd import kernels as K
+    say("sinkhorn kernels n", n, "m", m, "p", p)
+    rng = np.random.default_rng(n + 3 * m)
+    x, y = rng.uniform(size=(n, 3)), rng.uniform(size=(m, 3))
+    marg = rng.uniform(0.1, 1, n)
+    marg /= marg.sum()
+    phi, psi = rng.normal(0, 3, n), rng.normal(0, 3, m)
+    winf = float(oracle.max_cost_rows(x, y, p).max())
+    reg = 0.03
+    ref = oracle.sinkhorn_half_step(x, y, marg, psi, reg, winf, p)
+    dx, dy, dm, dphi, dpsi = (dev(gpu_ctx, a) for a in (x, y, marg, phi, psi))
+    o = out(gpu_ctx, n)
+    K.warp_logsumexp_ct_fused(gpu_ctx, o, dx, dy, dm, dpsi, reg, winf, p, FP64)
+    assert rel(o.get(), ref) < FP64_REL
+    K.warp_logsumexp_ct_fused(gpu_ctx, o, dx, dy, dm, dpsi, reg, winf, p, MIXED)
+    vmax = 1.0 / reg + np.max(np.abs(psi))
+    assert np.max(np.abs(o.get() - ref)) < MIXED_LSE_ULPS * vmax + 1e-9
+    pa = 1.0 if p == 1 else 2.0
+    res_ref, cost_ref = oracle.row_residual_cost(x, y, marg, phi, psi, reg, winf, pa)
+    res, cost = out(gpu_ctx, n), out(gpu_ctx, n)
+    K.residual_opt(gpu_ctx, res, cost, dx, dy, dm, dphi, dpsi, reg, winf, p, FP64)
+    scale = np.maximum(np.abs(res_ref), marg)
+    assert np.max(np.abs(res.get() - res_ref) / scale) < 1e-10
+    assert rel(cost.get(), cost_ref, 1e-300) < 1e-10
+    b_ref = oracle.barycentric_sinkhorn(x, y, phi, psi, reg, winf, pa)
+    b = out(gpu_ctx, 3 * n)
+    K.naive_findmaxindex_ct(gpu_ctx, b, dx, dy, dphi, dpsi, reg, winf, p)
+    assert np.max(np.abs(b.get().reshape(-1, 3) - b_ref)) < 1e-10
+
+
+def test_edge_cases(gpu_ctx, oracle):
+    from culamp_amd import FP64, MIXED
+    from culamp_amd import kernels as K
+    x, y, th, r = _inputs(5, 9, 6)
+    dx, dy, dth = dev(gpu_ctx, x), dev(gpu_ctx, y), dev(gpu_ctx, th)
+    o = out(gpu_ctx, 9)
+    # reg = Inf (first LAMP iteration): log m exactly
+    for prec in (FP64, MIXED):
+        K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, dy, dth, math.inf, 1.0, 2.0, 2.0, prec)
+        assert np.allclose(o.get(), np.log(6), rtol=1e-12, atol=0)
+    # m = 0: empty log-sum-exp is -Inf
+    import culamp_amd
+    empty = culamp_amd.DeviceArray(gpu_ctx, 0)
+    K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, empty, empty, 0.1, 1.0, 2.0, 2.0, FP64)
+    assert np.all(np.isneginf(o.get()))
+    # n = 0 is a no-op
+    K.warp_logsumexp_spp_ct_fused(gpu_ctx, empty, empty, dy, dth, 0.1, 1.0, 2.0, 2.0, FP64)
+    # invalid precision is reported, not run
+    with pytest.raises(culamp_amd.CulampError):
+        K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, dy, dth, 0.1, 1.0, 2.0, 2.0, 7)
+    # huge potentials: no overflow in either precision
+    big = dev(gpu_ctx, np.array([-1e3, 5.0, 1e3, 0.0, 2.0, -7.0]))
+    for prec in (FP64, MIXED):
+        K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, dy, big, 0.01, 1.0, 2.0, 2.0, prec)
+        ref = oracle.lse_rows(x, y, big.get(), 0.01, 1.0, 2.0, 2.0)
+        assert np.all(np.isfinite(o.get()))
+        assert np.max(np.abs(o.get() - ref) / np.abs(ref)) < (1e-12 if prec == FP64 else 1e-6)

@@ -1,0 +1,214 @@
+"""The FP64 oracle pinned against (a) the mpmath golden vectors and (b)
+closed forms / invariants of the reference's algorithms.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+from scipy.optimize import linprog
+
+from conftest import pval, unhex
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def test_golden_lamp_kernels(golden, oracle):
+    n_cases = 0
+    for c in golden:
+        if c["kind"] != "lamp":
+            continue
+        x, y = unhex(c["x"], (-1, 3)), unhex(c["y"], (-1, 3))
+        th, r = unhex(c["theta"]), unhex(c["r"])
+        p, reg = pval(c["p"]), pval(c["reg"])
+        logZ = oracle.lse_rows(x, y, th, reg, c["st"], c["winf"], p)
+        assert rel(logZ, c["logZ"]) < 1e-13, c["p"]
+        res, cost = oracle.col_marginal_cost(x, y, r, th, unhex(c["logZ_in"]), reg, c["st"], c["winf"], p)
+        assert rel(res, c["res"]) < 1e-12
+        assert rel(cost, c["cost"]) < 1e-12
+        assert rel(oracle.minplus_rows(x, y, th, c["winf"], p), c["minplus"]) < 1e-13
+        assert rel(oracle.max_cost_rows(x, y, p), c["maxcost"]) < 1e-14
+        if "bary_rows" in c:
+            b1 = oracle.barycentric_lamp_rows(x, y, th, r, unhex(c["logZ_in"]), reg, 1.0, c["winf"], p)
+            b2 = oracle.barycentric_lamp_cols(x, y, r, th, unhex(c["logZ_in"]), reg, 1.0, c["winf"], p)
+            assert rel(b1, c["bary_rows"]) < 1e-12
+            assert rel(b2, c["bary_cols"]) < 1e-12
+        n_cases += 1
+    assert n_cases == 12
+
+
+def test_golden_sinkhorn_kernels(golden, oracle):
+    n_cases = 0
+    for c in golden:
+        if c["kind"] != "sinkhorn":
+            continue
+        x, y = unhex(c["x"], (-1, 3)), unhex(c["y"], (-1, 3))
+        marg, phi, psi = unhex(c["marg"]), unhex(c["phi"]), unhex(c["psi"])
+        p = pval(c["p"])
+        half = oracle.sinkhorn_half_step(x, y, marg, psi, c["reg"], c["winf"], p)
+        assert rel(half, c["half"]) < 1e-13
+        res, cost = oracle.row_residual_cost(x, y, marg, phi, psi, c["reg"], c["winf"], p)
+        assert np.max(np.abs(res - np.array(c["resid"]))) < 1e-12 * max(1.0, np.max(np.abs(c["resid"])))
+        assert rel(cost, c["rcost"]) < 1e-12
+        assert rel(oracle.barycentric_sinkhorn(x, y, phi, psi, c["reg"], c["winf"], p), c["bary"]) < 1e-12
+        n_cases += 1
+    assert n_cases == 8
+
+
+def test_golden_theta_update(golden, oracle):
+    for c in golden:
+        if c["kind"] != "theta":
+            continue
+        out = oracle.theta_update(unhex(c["theta0"]), unhex(c["res"]), unhex(c["c"]), unhex(c["etamu"]),
+                                  c["eta_mu"], c["adjust"], c["minv"], c["maxv"], 1.0)
+        assert rel(out, c["out"]) < 1e-13
+
+
+# ---- closed forms / invariants ------------------------------------------------
+
+def grid(g, scale=1.0):
+    k = np.arange(g * g)
+    return np.stack([(k // g) * scale, (k % g) * scale, np.zeros(g * g)], axis=1).astype(float)
+
+
+@pytest.mark.parametrize("p,expect", [(2.0, 2 * 31 ** 2), (1.0, 2 * 31), (math.inf, 31.0)])
+def test_grid_winf_closed_form(oracle, p, expect):
+    """W_inf on the 32x32 integer grid: 2(g-1)^2, 2(g-1), g-1."""
+    x = grid(32)
+    assert oracle.max_cost_rows(x, x, p).max() == expect
+    # dense get_euclidean_distance agrees (src/Utilities.jl:145-157, p >= 10 is Chebyshev)
+    W = oracle.grid_cost_matrix(32, 32, p if p != math.inf else 10.0)
+    assert W.max() == expect
+
+
+def test_lse_identities(oracle):
+    rng = np.random.default_rng(1)
+    x, y = rng.uniform(size=(6, 3)), rng.uniform(size=(11, 3))
+    th = rng.uniform(-1, 1, 11)
+    base = oracle.lse_rows(x, y, th, 0.07, 1.0, 1.3, 2.0)
+    # theta + kappa shifts the result by -kappa / reg
+    sh = oracle.lse_rows(x, y, th + 0.25, 0.07, 1.0, 1.3, 2.0)
+    assert np.allclose(sh, base - 0.25 / 0.07, rtol=0, atol=1e-12)
+    # st = 0 removes the cost: log sum_j exp(-theta_j / reg)
+    z = oracle.lse_rows(x, y, th, 0.07, 0.0, 1.3, 2.0)
+    ref = np.log(np.sum(np.exp(-th / 0.07)))
+    assert np.allclose(z, ref, rtol=1e-14)
+    # reg = inf (first LAMP iteration): log m
+    assert np.allclose(oracle.lse_rows(x, y, th, math.inf, 1.0, 1.3, 2.0), np.log(11), rtol=1e-15)
+    # m = 1: the exponent itself
+    one = oracle.lse_rows(x, y[:1], th[:1], 0.07, 1.0, 1.3, 2.0)
+    c0 = np.sum((x - y[0]) ** 2, axis=1)
+    assert np.allclose(one, -(c0 / 2.6 + th[0]) / 0.07, rtol=1e-13)
+
+
+def test_col_marginal_mass_invariant(oracle):
+    """With logZ the row LSE, sum_j res_j = sum_i r_i (rows normalised)."""
+    rng = np.random.default_rng(2)
+    x, y = rng.uniform(size=(40, 3)), rng.uniform(size=(33, 3))
+    th = rng.uniform(-1, 1, 33)
+    r = rng.uniform(0.1, 1, 40)
+    r /= r.sum()
+    for p in (1.0, 2.0, math.inf):
+        logZ = oracle.lse_rows(x, y, th, 0.03, 1.0, 2.0, p)
+        res, cost = oracle.col_marginal_cost(x, y, r, th, logZ, 0.03, 1.0, 2.0, p)
+        assert abs(res.sum() - 1.0) < 1e-13
+        assert np.all(cost >= 0)
+
+
+def test_sinkhorn_matches_dense_and_lp(oracle):
+    """Kernel Sinkhorn (on-the-fly, relative reg) and dense sinkhorn_log on the
+    normalised W reach the same unique entropic fixed point; small reg
+    approaches the LP optimum (scipy HiGHS)."""
+    g = 4
+    x = grid(g)
+    rng = np.random.default_rng(3)
+    r = rng.uniform(size=g * g) + 1e-3
+    r /= r.sum()
+    c = rng.uniform(size=g * g) + 1e-3
+    c /= c.sum()
+    eta = 0.05
+    sk = oracle.sinkhorn_solve(x, x, r, c, frequency=1, p=2.0, eta_p=eta, epsilon=6e-12,
+                               itermax=200000, verbose=False)
+    W = oracle.grid_cost_matrix(g, g, 2.0)
+    dense = oracle.sinkhorn_log_dense(r, c, W / W.max(), frequency=1, eta_p=eta, epsilon=6e-12,
+                                      itermax=200000, verbose=False)
+    P_k = np.exp(-W / W.max() / eta + sk["phi"][:, None] + sk["psi"][None, :])
+    assert np.allclose(P_k, dense["P"], atol=1e-10)
+    assert np.allclose(P_k.sum(1), r, atol=1e-10) and np.allclose(P_k.sum(0), c, atol=1e-12)
+    # transport cost in raw units vs the LP optimum
+    n = g * g
+    A = np.zeros((2 * n, n * n))
+    for i in range(n):
+        A[i, i * n:(i + 1) * n] = 1
+        A[n + i, i::n] = 1
+    lp = linprog(W.ravel(), A_eq=A, b_eq=np.concatenate([r, c]), bounds=(0, None), method="highs")
+    sk_small = oracle.sinkhorn_solve(x, x, r, c, frequency=10, p=2.0, eta_p=2e-3, epsilon=1e-9,
+                                     itermax=400000, verbose=False)
+    assert abs(sk_small["objective"] - lp.fun) / lp.fun < 0.05
+    assert sk_small["objective"] >= lp.fun * (1 - 1e-9)
+
+
+def test_lamp_converges_to_lp(oracle):
+    """LAMP (MirrorProxKernel.jl:1019-1132) with small eta_p reaches the stop
+    test and its transport cost approaches the LP optimum."""
+    g = 4
+    x = grid(g, 1.0 / (g - 1))
+    rng = np.random.default_rng(4)
+    r = rng.uniform(size=g * g) + 1e-3
+    r /= r.sum()
+    c = rng.uniform(size=g * g) + 1e-3
+    c /= c.sum()
+    # eta_p = 0: the reference's own stop test (primal - dual and infeasibility)
+    out = oracle.lamp_solve(x, x, r, c, frequency=10, p=2.0, eta_p=0.0, epsilon=6e-6, B=1.1,
+                            itermax=200000, verbose=True)
+    assert out["converged"] == 1
+    assert out["residual"] < 1e-6 and out["primal"] - out["dual"] < 1e-6
+    # eta_p > 0: infeasibility stop
+    out2 = oracle.lamp_solve(x, x, r, c, frequency=10, p=2.0, eta_p=1e-3, epsilon=6e-6, B=1.1,
+                             itermax=200000, verbose=False, stop="infeas")
+    assert out2["converged"] == 1 and out2["residual"] < 1e-6
+    n = g * g
+    W = np.sum((x[:, None, :] - x[None, :, :]) ** 2, axis=2)
+    A = np.zeros((2 * n, n * n))
+    for i in range(n):
+        A[i, i * n:(i + 1) * n] = 1
+        A[n + i, i::n] = 1
+    lp = linprog(W.ravel(), A_eq=A, b_eq=np.concatenate([r, c]), bounds=(0, None), method="highs")
+    assert abs(out["objective"] - lp.fun) / lp.fun < 1e-3
+    assert abs(out2["objective"] - lp.fun) / lp.fun < 0.05
+    # log format: header + rows with 7 fields ending in lamp_kernel
+    assert out["log"][0] == "time(s),iter,infeas,ot_objective,primal,dual,solver\n"
+    assert all(l.rstrip("\n").split(",")[-1] == "lamp_kernel" and len(l.split(",")) == 7 for l in out["log"][1:])
+
+
+def test_theta_update_clamp_bounds(oracle):
+    m = 50
+    rng = np.random.default_rng(5)
+    out = oracle.theta_update(rng.uniform(-0.99, 0.99, m), rng.uniform(0, 10, m), np.zeros(m),
+                              np.full(m, 1e-3), 0.0, 1, math.tanh(-0.55), math.tanh(0.55), 1.0)
+    assert np.all(out <= math.tanh(0.55)) and np.all(out >= math.tanh(-0.55))
+
+
+def test_annealing_schedule(oracle):
+    """eta_k = max(q^k eta_0, eps): stages and final eta (AnnealedSinkhorn.jl:79,128)."""
+    g = 3
+    x = grid(g)
+    r = np.full(g * g, 1.0 / (g * g))
+    out = oracle.annealed_sinkhorn_solve(x, x, r, r, frequency=5, p=2.0, eta_p=0.1, epsilon=0.02,
+                                         anneal_mult=0.5, itermax=10000, verbose=True)
+    etas = [float(l.split(",")[4]) for l in out["log"][1:]]
+    assert etas == [0.1, 0.05, 0.025, 0.02]
+    assert out["eta_t"] == 0.02
+
+
+def test_thread_count_invariance(oracle):
+    rng = np.random.default_rng(6)
+    x, y = rng.uniform(size=(300, 3)), rng.uniform(size=(200, 3))
+    th = rng.uniform(-1, 1, 200)
+    oracle.set_threads(1)
+    a = oracle.lse_rows(x, y, th, 0.01, 1.0, 3.0, 2.0)
+    oracle.set_threads(4)
+    b = oracle.lse_rows(x, y, th, 0.01, 1.0, 3.0, 2.0)
+    oracle.set_threads(1)
+    assert np.array_equal(a, b)
