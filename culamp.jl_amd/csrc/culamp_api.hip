@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -30,13 +31,12 @@ struct culamp_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double* partial = nullptr;
   size_t partial_cap = 0;  // doubles
-  unsigned* counters = nullptr;
-  size_t counter_cap = 0;
   double* scal = nullptr;      // device scalars for reductions
   double* scal_host = nullptr; // pinned mirror
   int force_rpt = 0, force_splits = 0;
   double pairs = 0.0;          // pair evaluations issued (throughput accounting)
   int64_t passes = 0;
+  std::set<culamp_problem*> problems;  // destroyed with the context
 };
 
 struct culamp_problem {
@@ -45,6 +45,16 @@ struct culamp_problem {
   double *x = nullptr, *y = nullptr, *r = nullptr, *c = nullptr;
   double *logr = nullptr, *logc = nullptr;
   std::vector<double> hr, hc;
+  // feature problems (cell-similarity costs); metric < 0 for point clouds
+  int metric = -1;
+  int64_t d = 0;
+  double scale = 1.0;
+  double *xf64 = nullptr, *yf64 = nullptr, *xrc = nullptr, *yrc = nullptr;
+  float *xf32 = nullptr, *yf32 = nullptr;
+  bool self = false;  // y is x
+  // MFMA path (cosine / correlation): unit rows in bf16, width 16 * zks
+  uint16_t *xz_hi = nullptr, *xz_lo = nullptr, *yz_hi = nullptr, *yz_lo = nullptr;
+  int zks = 0;  // 0: not eligible
 };
 
 namespace {
@@ -92,7 +102,7 @@ void emit(culamp_log_fn fn, void* user, const std::string& s) {
   if (fn) fn(user, s.c_str());
 }
 
-int ensure_work(culamp_ctx* ctx, size_t partial_doubles, size_t counters) {
+int ensure_work(culamp_ctx* ctx, size_t partial_doubles) {
   if (partial_doubles > ctx->partial_cap) {
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ctx->partial) HIP_TRY(hipFree(ctx->partial));
@@ -101,29 +111,47 @@ int ensure_work(culamp_ctx* ctx, size_t partial_doubles, size_t counters) {
     HIP_TRY(hipMalloc(&ctx->partial, cap * sizeof(double)));
     ctx->partial_cap = cap;
   }
-  if (counters > ctx->counter_cap) {
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (ctx->counters) HIP_TRY(hipFree(ctx->counters));
-    ctx->counters = nullptr;
-    size_t cap = counters + 1024;
-    HIP_TRY(hipMalloc(&ctx->counters, cap * sizeof(unsigned)));
-    HIP_TRY(hipMemsetAsync(ctx->counters, 0, cap * sizeof(unsigned), ctx->stream));
-    ctx->counter_cap = cap;
-  }
   return CULAMP_OK;
 }
 
 int run_pass(culamp_ctx* ctx, const PassDesc& d, int precision, int kind) {
   if (d.nt <= 0) return CULAMP_OK;
-  if (precision != CULAMP_FP64 && precision != CULAMP_MIXED)
-    return fail(CULAMP_ERR_INVALID_ARG, "precision must be CULAMP_FP64 or CULAMP_MIXED");
+  if (precision < CULAMP_FP64 || precision > CULAMP_BF16X3)
+    return fail(CULAMP_ERR_INVALID_ARG, "precision must be CULAMP_FP64, _MIXED, _BF16 or _BF16X3");
+  if (d.metric >= 0 && (precision == CULAMP_BF16 || precision == CULAMP_BF16X3) && d.zks > 0 &&
+      (kind == KIND_LSE || kind == KIND_SUM)) {
+    const int nsplit = precision == CULAMP_BF16X3 ? 3 : 1;
+    LaunchCfg cfg = choose_mfma_launch(d.nt, d.ns, ctx->num_cu, ctx->force_splits);
+    if (cfg.splits > 1) TRY(ensure_work(ctx, (size_t)pass_fields(kind) * cfg.splits * cfg.tblocks * 128));
+    Work w{ctx->partial};
+    HIP_TRY(launch_feature_mfma(d, kind, nsplit, cfg, w, ctx->stream));
+    ctx->pairs += (double)d.nt * (double)d.ns;
+    ctx->passes += 1;
+    return CULAMP_OK;
+  }
+  if (precision > CULAMP_MIXED) precision = CULAMP_MIXED;
+  if (d.metric >= 0) {
+    if (kind == KIND_BARY) return fail(CULAMP_ERR_INVALID_ARG, "no barycentric pass for feature costs");
+    const bool fp32 = precision == CULAMP_MIXED && (kind == KIND_LSE || kind == KIND_SUM);
+    LaunchCfg cfg = choose_feature_launch(d.nt, d.ns, ctx->num_cu, ctx->force_splits);
+    if (cfg.splits > 1) {
+      const size_t ntp = (size_t)cfg.tblocks * kFeatTargets;
+      TRY(ensure_work(ctx, (size_t)pass_fields(kind) * cfg.splits * ntp));
+    }
+    Work w{ctx->partial};
+    HIP_TRY(fp32 ? launch_feature_fp32(d, kind, cfg, w, ctx->stream)
+                 : launch_feature_fp64(d, kind, cfg, w, ctx->stream));
+    ctx->pairs += (double)d.nt * (double)d.ns;
+    ctx->passes += 1;
+    return CULAMP_OK;
+  }
   LaunchCfg cfg = choose_launch(d.nt, d.ns, precision, kind, ctx->num_cu,
                                 ctx->force_rpt, ctx->force_splits);
   if (cfg.splits > 1) {
     const size_t ntp = (size_t)cfg.tblocks * kThreads * cfg.rpt;
-    TRY(ensure_work(ctx, (size_t)pass_fields(kind) * cfg.splits * ntp, (size_t)cfg.tblocks));
+    TRY(ensure_work(ctx, (size_t)pass_fields(kind) * cfg.splits * ntp));
   }
-  Work w{ctx->partial, ctx->counters};
+  Work w{ctx->partial};
   HIP_TRY(launch_pass(d, precision, kind, cfg, w, ctx->stream));
   ctx->pairs += (double)d.nt * (double)d.ns;
   ctx->passes += 1;
@@ -133,7 +161,32 @@ int run_pass(culamp_ctx* ctx, const PassDesc& d, int precision, int kind) {
 PassDesc blank() {
   PassDesc d;
   std::memset(&d, 0, sizeof d);
+  d.metric = -1;
   return d;
+}
+
+// Point the pass at a feature problem's packed rows: `rows` makes the rows
+// (x side) the targets, otherwise the columns (y side).
+void feat(PassDesc& d, const culamp_problem* pb, bool rows) {
+  if (pb->metric < 0) return;
+  d.metric = pb->metric;
+  d.dim = pb->d;
+  d.fscale = pb->scale;
+  d.tgt = d.src = nullptr;
+  if (rows) {
+    d.nt = pb->n; d.ns = pb->m;
+    d.tf64 = pb->xf64; d.sf64 = pb->yf64; d.tf32 = pb->xf32; d.sf32 = pb->yf32;
+    d.trc = pb->xrc; d.src_rc = pb->yrc;
+    d.tz_hi = (const __bf16*)pb->xz_hi; d.tz_lo = (const __bf16*)pb->xz_lo;
+    d.sz_hi = (const __bf16*)pb->yz_hi; d.sz_lo = (const __bf16*)pb->yz_lo;
+  } else {
+    d.nt = pb->m; d.ns = pb->n;
+    d.tf64 = pb->yf64; d.sf64 = pb->xf64; d.tf32 = pb->yf32; d.sf32 = pb->xf32;
+    d.trc = pb->yrc; d.src_rc = pb->xrc;
+    d.tz_hi = (const __bf16*)pb->yz_hi; d.tz_lo = (const __bf16*)pb->yz_lo;
+    d.sz_hi = (const __bf16*)pb->xz_hi; d.sz_lo = (const __bf16*)pb->xz_lo;
+  }
+  d.zks = pb->zks;
 }
 
 int check_ctx(culamp_ctx* ctx) {
@@ -274,6 +327,24 @@ int device_winf(culamp_ctx* ctx, const double* x, int64_t n, const double* y,
   return CULAMP_OK;
 }
 
+int reduce_to_host(culamp_ctx* ctx, const ReduceSpec& rs, double* out);
+
+// W_inf of a problem: point clouds as max_logsumexp_spp_ct!
+// (MirrorProxKernel.jl:280-326); feature problems as max_logsumexp_cellsim!
+// (MirrorProxCellSim.jl:1-41), whose running max starts at 0.
+int problem_winf(culamp_ctx* ctx, culamp_problem* pb, double p, double* tmp_n, double* winf) {
+  if (pb->metric < 0) return device_winf(ctx, pb->x, pb->n, pb->y, pb->m, p, tmp_n, winf);
+  PassDesc d = max_desc(nullptr, pb->n, nullptr, pb->m, p, tmp_n);
+  feat(d, pb, true);
+  TRY(run_pass(ctx, d, CULAMP_FP64, KIND_MAX));
+  ReduceSpec rs{};
+  rs.nops = 1; rs.op[0] = 3; rs.a[0] = tmp_n; rs.len[0] = pb->n;
+  double v[1];
+  TRY(reduce_to_host(ctx, rs, v));
+  *winf = std::fmax(v[0], 0.0);
+  return CULAMP_OK;
+}
+
 int reduce_to_host(culamp_ctx* ctx, const ReduceSpec& rs, double* out) {
   HIP_TRY(launch_reduce(rs, ctx->scal, ctx->stream));
   HIP_TRY(hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(double) * rs.nops, hipMemcpyDeviceToHost, ctx->stream));
@@ -314,31 +385,46 @@ struct SkState {
 
 // One sinkhorn_color_transfer solve on device-resident state
 // (SinkhornKernel.jl:477-559); phi/psi hold the warm start on entry.
+// For feature problems this is sinkhorn_cellsim (SinkhornCellSim.jl:240-310).
 int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
                   int64_t frequency, double p, culamp_log_fn log_fn, void* user,
                   SkState& st, culamp_result* out) {
   const int64_t n = pb->n, m = pb->m;
+  const bool cellsim = pb->metric >= 0;
   const double t0 = now_s();
   const double eta = a->eta_p;
   double winf;
-  TRY(device_winf(ctx, pb->x, n, pb->y, m, p, st.res, &winf));
+  TRY(problem_winf(ctx, pb, p, st.res, &winf));
   const double p_lse = (p == INFINITY) ? sk_aux_p(p) : p;
   const double p_aux = sk_aux_p(p);
-  const int prec = a->precision;
+  const char* tag = cellsim ? ",sinkhorn_cellsim\n" : ",sinkhorn_kernel\n";
+  auto half_rows = [&]() {
+    PassDesc d1 = sk_half_desc(pb->x, n, pb->y, m, pb->r, st.psi, eta, winf, p_lse, st.phi);
+    feat(d1, pb, true);
+    d1.prev_offset = 1;
+    return run_pass(ctx, d1, a->precision, KIND_LSE);
+  };
+  auto half_cols = [&]() {
+    PassDesc d2 = sk_half_desc(pb->y, m, pb->x, n, pb->c, st.phi, eta, winf, p_lse, st.psi);
+    feat(d2, pb, false);
+    d2.prev_offset = 1;
+    return run_pass(ctx, d2, a->precision, KIND_LSE);
+  };
+  auto resid = [&]() {
+    PassDesc d3 = sk_resid_desc(pb->x, n, pb->y, m, pb->r, st.phi, st.psi, eta, winf, p_aux, st.res, st.cost);
+    feat(d3, pb, true);
+    return run_pass(ctx, d3, a->precision, KIND_SUM);
+  };
   if (a->verbose) emit(log_fn, user, "time(s),iter,infeas,ot_objective,dual\n");
   int64_t num_iter = 0;
   int converged = 0;
   for (int64_t it = 1; it <= a->itermax; ++it) {
     const double elapsed = now_s() - t0;
     if (elapsed > a->tmax) break;
-    PassDesc d1 = sk_half_desc(pb->x, n, pb->y, m, pb->r, st.psi, eta, winf, p_lse, st.phi);
-    d1.prev_offset = 1;
-    TRY(run_pass(ctx, d1, prec, KIND_LSE));
-    PassDesc d2 = sk_half_desc(pb->y, m, pb->x, n, pb->c, st.phi, eta, winf, p_lse, st.psi);
-    d2.prev_offset = 1;
-    TRY(run_pass(ctx, d2, prec, KIND_LSE));
+    TRY(half_rows());
+    TRY(half_cols());
     if ((it - 1) % frequency == 0) {
-      TRY(run_pass(ctx, sk_resid_desc(pb->x, n, pb->y, m, pb->r, st.phi, st.psi, eta, winf, p_aux, st.res, st.cost), prec, KIND_SUM));
+      TRY(resid());
       ReduceSpec rs{};
       rs.nops = 4;
       rs.op[0] = 5; rs.a[0] = st.res; rs.len[0] = n;
@@ -351,7 +437,7 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
       if (a->verbose) {
         emit(log_fn, user, fmt("%.6e", elapsed) + "," + std::to_string(it) + "," +
                                fmt("%.14e", rr) + "," + fmt("%.14e", v[1]) + "," +
-                               fmt("%.14e", v[2] + v[3]) + ",sinkhorn_kernel\n");
+                               fmt("%.14e", v[2] + v[3]) + tag);
       }
       if (rr <= a->epsilon / 6.0) {
         converged = 1;
@@ -360,7 +446,7 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
     }
     num_iter += 1;
   }
-  TRY(run_pass(ctx, sk_resid_desc(pb->x, n, pb->y, m, pb->r, st.phi, st.psi, eta, winf, p_aux, st.res, st.cost), prec, KIND_SUM));
+  TRY(resid());
   ReduceSpec rs{};
   rs.nops = 4;
   rs.op[0] = 5; rs.a[0] = st.res; rs.len[0] = n;
@@ -443,8 +529,8 @@ int culamp_ctx_destroy(culamp_ctx* ctx) {
   if (!ctx) return CULAMP_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  while (!ctx->problems.empty()) culamp_problem_destroy(*ctx->problems.begin());
   if (ctx->partial) (void)hipFree(ctx->partial);
-  if (ctx->counters) (void)hipFree(ctx->counters);
   if (ctx->scal) (void)hipFree(ctx->scal);
   if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -641,18 +727,263 @@ int culamp_problem_create(culamp_ctx* ctx, const double* x, int64_t n,
     culamp_problem_destroy(pb);
     return fail(CULAMP_ERR_HIP, "problem upload failed: %s", hipGetErrorString(e));
   }
+  ctx->problems.insert(pb);
   *out = pb;
   return CULAMP_OK;
 }
 
 int culamp_problem_destroy(culamp_problem* pb) {
   if (!pb) return CULAMP_OK;
+  pb->ctx->problems.erase(pb);
   (void)hipSetDevice(pb->ctx->device);
   (void)hipStreamSynchronize(pb->ctx->stream);
-  for (double* p : {pb->x, pb->y, pb->r, pb->c, pb->logr, pb->logc})
+  for (double* p : {pb->x, pb->y, pb->r, pb->c, pb->logr, pb->logc, pb->xf64, pb->xrc})
     if (p) (void)hipFree(p);
+  if (pb->xf32) (void)hipFree(pb->xf32);
+  for (uint16_t* p : {pb->xz_hi, pb->xz_lo})
+    if (p) (void)hipFree(p);
+  if (!pb->self) {
+    for (double* p : {pb->yf64, pb->yrc})
+      if (p) (void)hipFree(p);
+    if (pb->yf32) (void)hipFree(pb->yf32);
+    for (uint16_t* p : {pb->yz_hi, pb->yz_lo})
+      if (p) (void)hipFree(p);
+  }
   delete pb;
   return CULAMP_OK;
+}
+
+// get_flat_metrics (src/CellUtilities.jl:304-311) on an n x d column-major
+// matrix, then the packed forms the feature kernel reads: rows divided by
+// their sums, and [squared norm / s^2 | mean / s | centred squared norm / s^2].
+static void pack_features(const double* x, int64_t n, int64_t d, bool scaled,
+                          std::vector<double>& f64, std::vector<float>& f32,
+                          std::vector<double>& rc) {
+  f64.resize((size_t)n * d);
+  f32.resize((size_t)n * d);
+  rc.resize((size_t)4 * n);
+  for (int64_t i = 0; i < n; ++i) {
+    double s = 0.0, q = 0.0;
+    for (int64_t k = 0; k < d; ++k) {
+      const double v = x[i + k * n];
+      s += v;
+      q += v * v;
+    }
+    const double mu = s / (double)d;
+    double cs = 0.0;
+    for (int64_t k = 0; k < d; ++k) {
+      const double v = x[i + k * n] - mu;
+      cs += v * v;
+    }
+    for (int64_t k = 0; k < d; ++k) {
+      const double v = scaled ? x[i + k * n] / s : x[i + k * n];
+      f64[i + k * n] = v;
+      f32[i + k * n] = (float)v;
+    }
+    rc[i] = q / (s * s);
+    rc[n + i] = mu / s;
+    rc[2 * n + i] = cs / (s * s);
+    rc[3 * n + i] = s;
+  }
+}
+
+static uint16_t bf16_rne(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static float bf16_to_float(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// Unit rows for the MFMA path: z_i = sign(s_i) x_i / |x_i| (cosine) or
+// sign(s_i) (x_i - mean_i) / |x_i - mean_i| (correlation), whose dot is the
+// reference's clamped similarity.  Returns false when some row has no
+// direction (the reference's zero-denominator branches), or d > 128.
+static bool pack_unit_rows(const double* x, int64_t n, int64_t d, int metric, int ks,
+                           std::vector<uint16_t>& hi, std::vector<uint16_t>& lo) {
+  const int64_t dp = 16 * ks;
+  hi.assign((size_t)n * dp, 0);
+  lo.assign((size_t)n * dp, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int64_t k = 0; k < d; ++k) s += x[i + k * n];
+    const double mu = metric == CULAMP_METRIC_CORRELATION ? s / (double)d : 0.0;
+    double q = 0.0;
+    for (int64_t k = 0; k < d; ++k) {
+      const double v = x[i + k * n] - mu;
+      q += v * v;
+    }
+    if (!(q > 0.0) || !(std::isfinite(s)) || s == 0.0) return false;
+    const double f = (s > 0 ? 1.0 : -1.0) / std::sqrt(q);
+    for (int64_t k = 0; k < d; ++k) {
+      const double z = (x[i + k * n] - mu) * f;
+      const uint16_t h = bf16_rne((float)z);
+      hi[(size_t)i * dp + k] = h;
+      lo[(size_t)i * dp + k] = bf16_rne((float)(z - (double)bf16_to_float(h)));
+    }
+  }
+  return true;
+}
+
+int culamp_feature_problem_create(culamp_ctx* ctx, const double* x, int64_t n,
+                                  const double* y, int64_t m, int64_t d, int metric,
+                                  double scale, const double* r, const double* c,
+                                  culamp_problem** out) {
+  TRY(check_ctx(ctx));
+  if (!out) return fail(CULAMP_ERR_INVALID_ARG, "null output pointer");
+  *out = nullptr;
+  if (!x || !r || !c) return fail(CULAMP_ERR_INVALID_ARG, "null input array");
+  const bool self = (y == nullptr || y == x);
+  if (self) m = n;
+  if (n <= 0 || m <= 0 || d <= 0)
+    return fail(CULAMP_ERR_INVALID_ARG, "feature problem sizes must be positive (n=%lld, m=%lld, d=%lld)",
+                (long long)n, (long long)m, (long long)d);
+  if (metric < CULAMP_METRIC_L1 || metric > CULAMP_METRIC_CORRELATION)
+    return fail(CULAMP_ERR_INVALID_ARG, "unknown metric %d", metric);
+  if (!(scale > 0.0)) return fail(CULAMP_ERR_INVALID_ARG, "scale must be positive");
+  culamp_problem* pb = new culamp_problem();
+  pb->ctx = ctx; pb->n = n; pb->m = m; pb->d = d; pb->metric = metric; pb->scale = scale;
+  pb->self = self;
+  pb->hr.assign(r, r + n);
+  pb->hc.assign(c, c + m);
+  std::vector<double> xf, xrc, yf, yrc;
+  std::vector<float> xf32, yf32;
+  // l1 reads x_ik / s_i per element; the dot metrics read raw rows and divide
+  // the dot by s_i s_j (CellUtilities.jl:213-220, 233-238)
+  const bool scaled = metric == CULAMP_METRIC_L1;
+  pack_features(x, n, d, scaled, xf, xf32, xrc);
+  if (!self) pack_features(y, m, d, scaled, yf, yf32, yrc);
+  hipError_t e = hipSuccess;
+  auto al = [&](void** p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(p, bytes); };
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+  };
+  al((void**)&pb->r, sizeof(double) * n); al((void**)&pb->c, sizeof(double) * m);
+  al((void**)&pb->logr, sizeof(double) * n); al((void**)&pb->logc, sizeof(double) * m);
+  al((void**)&pb->xf64, sizeof(double) * n * d); al((void**)&pb->xf32, sizeof(float) * n * d);
+  al((void**)&pb->xrc, sizeof(double) * 4 * n);
+  if (!self) {
+    al((void**)&pb->yf64, sizeof(double) * m * d); al((void**)&pb->yf32, sizeof(float) * m * d);
+    al((void**)&pb->yrc, sizeof(double) * 4 * m);
+  }
+  up(pb->r, r, sizeof(double) * n); up(pb->c, c, sizeof(double) * m);
+  up(pb->xf64, xf.data(), sizeof(double) * n * d); up(pb->xf32, xf32.data(), sizeof(float) * n * d);
+  up(pb->xrc, xrc.data(), sizeof(double) * 4 * n);
+  if (!self) {
+    up(pb->yf64, yf.data(), sizeof(double) * m * d); up(pb->yf32, yf32.data(), sizeof(float) * m * d);
+    up(pb->yrc, yrc.data(), sizeof(double) * 4 * m);
+  } else {
+    pb->yf64 = pb->xf64; pb->yf32 = pb->xf32; pb->yrc = pb->xrc;
+  }
+  std::vector<uint16_t> xzh, xzl, yzh, yzl;
+  if ((metric == CULAMP_METRIC_COSINE || metric == CULAMP_METRIC_CORRELATION) && d <= 128) {
+    int ks = 1;
+    while (16 * ks < d) ks *= 2;
+    bool ok = pack_unit_rows(x, n, d, metric, ks, xzh, xzl);
+    if (ok && !self) ok = pack_unit_rows(y, m, d, metric, ks, yzh, yzl);
+    if (ok) {
+      const size_t dp = (size_t)16 * ks;
+      pb->zks = ks;
+      al((void**)&pb->xz_hi, 2 * n * dp); al((void**)&pb->xz_lo, 2 * n * dp);
+      up(pb->xz_hi, xzh.data(), 2 * n * dp); up(pb->xz_lo, xzl.data(), 2 * n * dp);
+      if (!self) {
+        al((void**)&pb->yz_hi, 2 * m * dp); al((void**)&pb->yz_lo, 2 * m * dp);
+        up(pb->yz_hi, yzh.data(), 2 * m * dp); up(pb->yz_lo, yzl.data(), 2 * m * dp);
+      } else {
+        pb->yz_hi = pb->xz_hi; pb->yz_lo = pb->xz_lo;
+      }
+    }
+  }
+  if (e == hipSuccess) e = launch_log(pb->logr, pb->r, n, ctx->stream);
+  if (e == hipSuccess) e = launch_log(pb->logc, pb->c, m, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    culamp_problem_destroy(pb);
+    return fail(CULAMP_ERR_HIP, "feature problem upload failed: %s", hipGetErrorString(e));
+  }
+  ctx->problems.insert(pb);
+  *out = pb;
+  return CULAMP_OK;
+}
+
+static int need_features(const culamp_problem* pb) {
+  if (!pb) return fail(CULAMP_ERR_INVALID_ARG, "null problem");
+  if (pb->metric < 0) return fail(CULAMP_ERR_INVALID_ARG, "not a feature problem (use culamp_feature_problem_create)");
+  return CULAMP_OK;
+}
+
+// ---- feature kernel level (device vectors, problem-owned packed rows) -------
+
+// warp_logsumexp_spp_sim_fused! (src/solvers/MirrorProxCellSim.jl:259-342)
+int culamp_feature_lse_rows(culamp_ctx* ctx, culamp_problem* pb, const double* theta,
+                            double reg, double st, double winf, int precision, double* out) {
+  TRY(check_ctx(ctx));
+  TRY(need_features(pb));
+  PassDesc d = lse_rows_desc(nullptr, pb->n, nullptr, pb->m, theta, reg, st, winf, 2.0, out);
+  feat(d, pb, true);
+  return run_pass(ctx, d, precision, KIND_LSE);
+}
+
+// residual_cellsim_c! (MirrorProxCellSim.jl:190-257)
+int culamp_feature_col_marginal_cost(culamp_ctx* ctx, culamp_problem* pb, const double* r,
+                                     const double* theta, const double* logZ, double reg,
+                                     double st, double winf, int precision, double* res,
+                                     double* cost) {
+  TRY(check_ctx(ctx));
+  TRY(need_features(pb));
+  PassDesc d = colmarg_desc(nullptr, pb->n, nullptr, pb->m, r, theta, logZ, reg, st, winf, 2.0, res, cost);
+  feat(d, pb, false);
+  return run_pass(ctx, d, precision, KIND_SUM);
+}
+
+// warp_logsumexp_cellsim_fused! (src/solvers/SinkhornCellSim.jl:153-238):
+// rows != 0 updates the row potential from the column potential `pot`,
+// rows == 0 the column potential from the row potential.
+int culamp_feature_sinkhorn_half_step(culamp_ctx* ctx, culamp_problem* pb, int rows,
+                                      const double* marg, const double* pot, double reg,
+                                      double winf, int precision, double* out) {
+  TRY(check_ctx(ctx));
+  TRY(need_features(pb));
+  PassDesc d = sk_half_desc(nullptr, rows ? pb->n : pb->m, nullptr, rows ? pb->m : pb->n,
+                            marg, pot, reg, winf, 2.0, out);
+  feat(d, pb, rows != 0);
+  return run_pass(ctx, d, precision, KIND_LSE);
+}
+
+// residual_cellsim_opt! (SinkhornCellSim.jl:1-72)
+int culamp_feature_row_residual_cost(culamp_ctx* ctx, culamp_problem* pb, const double* marg,
+                                     const double* phi, const double* psi, double reg,
+                                     double winf, int precision, double* res, double* cost) {
+  TRY(check_ctx(ctx));
+  TRY(need_features(pb));
+  PassDesc d = sk_resid_desc(nullptr, pb->n, nullptr, pb->m, marg, phi, psi, reg, winf, 2.0, res, cost);
+  feat(d, pb, true);
+  return run_pass(ctx, d, precision, KIND_SUM);
+}
+
+// max_logsumexp_cellsim! (MirrorProxCellSim.jl:1-41): out_i = max(0, max_j C_ij)
+int culamp_feature_max_cost(culamp_ctx* ctx, culamp_problem* pb, double* out) {
+  TRY(check_ctx(ctx));
+  TRY(need_features(pb));
+  PassDesc d = max_desc(nullptr, pb->n, nullptr, pb->m, 2.0, out);
+  feat(d, pb, true);
+  return run_pass(ctx, d, CULAMP_FP64, KIND_MAX);
+}
+
+// warp_min_reduce_cellsim! (MirrorProxCellSim.jl:43-92)
+int culamp_feature_minplus_rows(culamp_ctx* ctx, culamp_problem* pb, const double* theta,
+                                double winf, double* out) {
+  TRY(check_ctx(ctx));
+  TRY(need_features(pb));
+  PassDesc d = minplus_desc(nullptr, pb->n, nullptr, pb->m, theta, winf, 2.0, out);
+  feat(d, pb, true);
+  return run_pass(ctx, d, CULAMP_FP64, KIND_MIN);
 }
 
 // ---- solvers ----------------------------------------------------------------
@@ -665,6 +996,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
                       culamp_result* out) {
   TRY(check_ctx(ctx));
   if (!pb || !a) return fail(CULAMP_ERR_INVALID_ARG, "null problem or args");
+  if (pb->metric >= 0) return fail(CULAMP_ERR_INVALID_ARG, "feature problem: use the cellsim solvers");
   if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
   const int64_t n = pb->n, m = pb->m;
   const int prec = a->precision;
@@ -817,6 +1149,7 @@ int culamp_sinkhorn_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args
                           double* img2, culamp_result* out) {
   TRY(check_ctx(ctx));
   if (!pb || !a) return fail(CULAMP_ERR_INVALID_ARG, "null problem or args");
+  if (pb->metric >= 0) return fail(CULAMP_ERR_INVALID_ARG, "feature problem: use the cellsim solvers");
   if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
   const int64_t n = pb->n, m = pb->m;
   const double pairs0 = ctx->pairs;
@@ -861,6 +1194,7 @@ int culamp_annealed_sinkhorn_solve(culamp_ctx* ctx, culamp_problem* pb,
                                    culamp_result* out) {
   TRY(check_ctx(ctx));
   if (!pb || !a_in) return fail(CULAMP_ERR_INVALID_ARG, "null problem or args");
+  if (pb->metric >= 0) return fail(CULAMP_ERR_INVALID_ARG, "feature problem: use the cellsim solvers");
   if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
   const int64_t n = pb->n, m = pb->m;
   culamp_args a = *a_in;
@@ -942,6 +1276,266 @@ int culamp_annealed_sinkhorn_solve(culamp_ctx* ctx, culamp_problem* pb,
     out->pairs = ctx->pairs - pairs0;
     out->passes = ctx->passes - passes0;
     out->nonfinite = !(std::isfinite(residual) && std::isfinite(objective));
+  }
+  return CULAMP_OK;
+}
+
+// ---- cell-similarity solvers ----------------------------------------------
+
+// extragradient_cellsim (src/solvers/MirrorProxCellSim.jl:94-188).  Differs
+// from the point-cloud LAMP loop: eta = eta_p (118), its own primal/dual
+// formulas (150-159), the averaged gamma for the eta = 0 dual (140, 179), and
+// phi = log r - (last row values computed), objective = sum of the last
+// column costs (184-187).
+int culamp_lamp_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
+                              int64_t frequency, culamp_log_fn log_fn, void* user,
+                              double* nu_out, double* phi_out, double* psi_out,
+                              culamp_result* out) {
+  TRY(check_ctx(ctx));
+  TRY(need_features(pb));
+  if (!a) return fail(CULAMP_ERR_INVALID_ARG, "null args");
+  if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
+  const int64_t n = pb->n, m = pb->m;
+  const int prec = a->precision;
+  const double pairs0 = ctx->pairs;
+  const int64_t passes0 = ctx->passes;
+  DevBufs B;
+  double *theta, *nu, *thetab, *nub, *gam, *res, *cost, *etamu, *sumvals;
+  TRY(B.alloc(&theta, m)); TRY(B.alloc(&nu, m)); TRY(B.alloc(&thetab, m));
+  TRY(B.alloc(&nub, m)); TRY(B.alloc(&gam, m)); TRY(B.alloc(&res, m));
+  TRY(B.alloc(&cost, m)); TRY(B.alloc(&etamu, m)); TRY(B.alloc(&sumvals, n > m ? n : m));
+  for (double* v : {theta, nu, thetab, nub, gam, res, cost}) HIP_TRY(launch_fill(v, 0.0, m, ctx->stream));
+  HIP_TRY(launch_fill(sumvals, 0.0, n, ctx->stream));
+  const double t0 = now_s();
+  double winf;
+  TRY(problem_winf(ctx, pb, 2.0, sumvals, &winf));
+  HIP_TRY(launch_fill(sumvals, 0.0, n, ctx->stream));
+  const double eta = a->eta_p;
+  HIP_TRY(launch_etamu(etamu, pb->c, a->alpha / (double)n, a->tau_mu, m, ctx->stream));
+  double negent = 0.0;
+  for (double v : pb->hr) if (v > 0.0) negent += v * std::log(v);
+  const double hr = eta * negent;
+  emit(log_fn, user, "time(s),iter,infeas,ot_objective,primal,dual,solver\n");
+  double st = 1.0, etat = INFINITY;
+  const double minv = std::tanh(-a->B / 2.0), maxv = std::tanh(a->B / 2.0);
+  double primal = NAN, dual = NAN, resid = NAN, objective = NAN;
+  int converged = 0;
+  int64_t steps = 0;
+  auto lse = [&](const double* pot, double reg, double s) {
+    PassDesc d = lse_rows_desc(nullptr, n, nullptr, m, pot, reg, s, winf, 2.0, sumvals);
+    feat(d, pb, true);
+    d.prev_offset = 1;
+    return run_pass(ctx, d, prec, KIND_LSE);
+  };
+  auto colmarg = [&](const double* pot, double reg, double s, PassDesc* proto) {
+    PassDesc d = colmarg_desc(nullptr, n, nullptr, m, pb->r, pot, sumvals, reg, s, winf, 2.0, res, cost);
+    if (proto) {
+      d.epi = EPI_LAMP_COL; d.fuse_update = 1; d.adjust = proto->adjust;
+      d.th0 = proto->th0; d.cvec = pb->c; d.etamu = etamu; d.eta_mu = a->eta_mu;
+      d.minv = minv; d.maxv = maxv; d.upd_winf = 1.0; d.th_out = proto->th_out;
+      d.nu_in = proto->nu_in; d.mix_src = proto->mix_src; d.nu_out = proto->nu_out;
+      d.mix_t = proto->mix_t;
+    }
+    feat(d, pb, false);
+    return run_pass(ctx, d, prec, KIND_SUM);
+  };
+  for (int64_t it = 1; it <= a->itermax; ++it) {
+    const double elapsed = now_s() - t0;
+    if (elapsed > a->tmax) break;
+    const bool check_it = a->verbose || a->stop == CULAMP_STOP_INFEAS;
+    if (check_it && (it - 1) % frequency == 0) {  // 144-166
+      TRY(lse(nu, etat, st));
+      TRY(colmarg(nu, etat, st, nullptr));
+      ReduceSpec rs{};
+      rs.nops = 4;
+      rs.op[0] = 1; rs.a[0] = res; rs.b[0] = pb->c; rs.len[0] = m;
+      rs.op[1] = 0; rs.a[1] = cost; rs.len[1] = m;
+      rs.op[2] = 2; rs.a[2] = pb->r; rs.b[2] = sumvals; rs.len[2] = n;
+      rs.op[3] = 2; rs.a[3] = pb->c; rs.b[3] = nu; rs.len[3] = m;
+      double v[4];
+      TRY(reduce_to_host(ctx, rs, v));
+      resid = v[0];
+      objective = v[1];
+      ReduceSpec rd{};
+      rd.nops = 2;
+      if (eta > 0) {
+        primal = objective * (1.0 - eta / etat) - eta * v[2] - 2.0 * winf * eta / etat * v[3] +
+                 hr + 2.0 * winf * resid;
+        TRY(lse(theta, eta, 1.0));
+        rd.op[1] = 2; rd.a[1] = pb->c; rd.b[1] = theta; rd.len[1] = m;
+      } else {
+        primal = objective + 2.0 * winf * resid;
+        PassDesc dm = minplus_desc(nullptr, n, nullptr, m, gam, winf, 2.0, sumvals);
+        feat(dm, pb, true);
+        TRY(run_pass(ctx, dm, CULAMP_FP64, KIND_MIN));
+        rd.op[1] = 2; rd.a[1] = pb->c; rd.b[1] = gam; rd.len[1] = m;
+      }
+      rd.op[0] = 2; rd.a[0] = pb->r; rd.b[0] = sumvals; rd.len[0] = n;
+      double w[2];
+      TRY(reduce_to_host(ctx, rd, w));
+      if (eta > 0) dual = -eta * w[0] - 2.0 * winf * w[1] + hr;
+      else dual = w[0] - 2.0 * winf * w[1];
+      if (a->verbose)
+        emit(log_fn, user, fmt("%.6e", elapsed) + "," + std::to_string(it) + "," +
+                               fmt("%.14e", resid) + "," + fmt("%.14e", objective) + "," +
+                               fmt("%.14e", primal) + "," + fmt("%.14e", dual) + ",lamp_kernel\n");
+      const bool met = a->stop == CULAMP_STOP_INFEAS
+                           ? resid < a->epsilon / 6.0
+                           : (primal - dual < a->epsilon / 6.0 && resid < a->epsilon / 6.0);
+      if (met) {
+        converged = 1;
+        break;
+      }
+    }
+    // 168-180
+    const double etat_new = 1.0 / (1.0 + (1.0 / etat) * (1.0 - eta));
+    TRY(lse(nu, etat, st));
+    {
+      PassDesc pr = blank();
+      pr.adjust = 0; pr.th0 = theta; pr.th_out = thetab;
+      pr.nu_in = nu; pr.mix_src = theta; pr.nu_out = nub; pr.mix_t = etat_new;
+      TRY(colmarg(nu, etat, st, &pr));
+    }
+    etat = etat_new;
+    st = (1.0 - etat) * st + etat;
+    TRY(lse(nub, etat, st));
+    {
+      PassDesc pr = blank();
+      pr.adjust = 1; pr.th0 = theta; pr.th_out = theta;
+      pr.nu_in = nu; pr.mix_src = thetab; pr.nu_out = nu; pr.mix_t = etat;
+      TRY(colmarg(nub, etat, st, &pr));
+    }
+    HIP_TRY(launch_axpby(gam, gam, 1.0 - etat, theta, etat, m, ctx->stream));
+    steps += 1;
+  }
+  ReduceSpec ro{};
+  ro.nops = 1; ro.op[0] = 0; ro.a[0] = cost; ro.len[0] = m;
+  double objf;
+  TRY(reduce_to_host(ctx, ro, &objf));
+  if (psi_out) {
+    HIP_TRY(launch_axpby(res, nu, -2.0 * winf / a->eta_p, nu, 0.0, m, ctx->stream));
+    TRY(copy_out(ctx, psi_out, res, m));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  if (phi_out) {
+    HIP_TRY(launch_log_minus(cost, pb->r, sumvals, n, ctx->stream));
+    TRY(copy_out(ctx, phi_out, cost, n));
+  }
+  TRY(copy_out(ctx, nu_out, nu, m));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (out) {
+    out->iterations = steps;
+    out->objective = objf;
+    out->residual = resid;
+    out->primal = primal;
+    out->dual = dual;
+    out->winf = winf;
+    out->eta_t = etat;
+    out->elapsed_s = now_s() - t0;
+    out->converged = converged;
+    out->pairs = ctx->pairs - pairs0;
+    out->passes = ctx->passes - passes0;
+    out->nonfinite = !std::isfinite(objf);
+  }
+  return CULAMP_OK;
+}
+
+// sinkhorn_cellsim (src/solvers/SinkhornCellSim.jl:240-310); the cost of a
+// pair is between row i of x and row j of y (the reference evaluates it
+// within one data matrix and is only called with data1 == data2).
+int culamp_sinkhorn_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
+                                  int64_t frequency, const double* phi0, const double* psi0,
+                                  culamp_log_fn log_fn, void* user, double* phi_out,
+                                  double* psi_out, culamp_result* out) {
+  TRY(check_ctx(ctx));
+  TRY(need_features(pb));
+  if (!a) return fail(CULAMP_ERR_INVALID_ARG, "null args");
+  if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
+  const int64_t n = pb->n, m = pb->m;
+  const double pairs0 = ctx->pairs;
+  const int64_t passes0 = ctx->passes;
+  DevBufs B;
+  SkState st;
+  TRY(B.alloc(&st.phi, n)); TRY(B.alloc(&st.psi, m));
+  TRY(B.alloc(&st.res, n > m ? n : m)); TRY(B.alloc(&st.cost, n));
+  if (phi0) HIP_TRY(hipMemcpyAsync(st.phi, phi0, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  else HIP_TRY(launch_fill(st.phi, 0.0, n, ctx->stream));
+  if (psi0) HIP_TRY(hipMemcpyAsync(st.psi, psi0, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+  else HIP_TRY(launch_fill(st.psi, 0.0, m, ctx->stream));
+  culamp_result r{};
+  TRY(sinkhorn_core(ctx, pb, a, frequency, 2.0, log_fn, user, st, &r));
+  TRY(copy_out(ctx, phi_out, st.phi, n));
+  TRY(copy_out(ctx, psi_out, st.psi, m));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  r.pairs = ctx->pairs - pairs0;
+  r.passes = ctx->passes - passes0;
+  if (out) *out = r;
+  return CULAMP_OK;
+}
+
+// annealed_sinkhorn_cellsim (src/solvers/AnnealedSinkhornCellSim.jl:1-63).
+// No tmax test between stages.  The stage solve updates the warm-start arrays
+// in place (SinkhornCellSim.jl:257-259), so after the extrapolation the
+// "previous" potentials are the extrapolated warm start (49-52).
+int culamp_annealed_sinkhorn_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb,
+                                           const culamp_args* a_in, int64_t frequency,
+                                           culamp_log_fn log_fn, void* user, double* phi_out,
+                                           double* psi_out, culamp_result* out) {
+  TRY(check_ctx(ctx));
+  TRY(need_features(pb));
+  if (!a_in) return fail(CULAMP_ERR_INVALID_ARG, "null args");
+  if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
+  const int64_t n = pb->n, m = pb->m;
+  const double pairs0 = ctx->pairs;
+  const int64_t passes0 = ctx->passes;
+  culamp_args a = *a_in;
+  const double eta_final = a.epsilon;
+  double eta = std::fmax(a.eta_p, eta_final);
+  const double t0 = now_s();
+  DevBufs B;
+  SkState st;
+  double *phiprev, *psiprev;
+  TRY(B.alloc(&st.phi, n)); TRY(B.alloc(&st.psi, m));
+  TRY(B.alloc(&st.res, n > m ? n : m)); TRY(B.alloc(&st.cost, n));
+  TRY(B.alloc(&phiprev, n)); TRY(B.alloc(&psiprev, m));
+  HIP_TRY(hipMemcpyAsync(st.phi, pb->logr, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(st.psi, pb->logc, sizeof(double) * m, hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(phiprev, pb->logr, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(psiprev, pb->logc, sizeof(double) * m, hipMemcpyDeviceToDevice, ctx->stream));
+  const double q = a.anneal_mult;
+  double preveta = 1.0;
+  const int verbose = a.verbose;
+  a.verbose = 0;
+  int64_t total = 0;
+  culamp_result r{};
+  if (verbose) emit(log_fn, user, "time(s),iter,infeas,ot_objective,eta,solver\n");
+  for (;;) {
+    a.eta_p = eta;
+    TRY(sinkhorn_core(ctx, pb, &a, frequency, 2.0, nullptr, nullptr, st, &r));
+    total += r.iterations;
+    if (verbose)
+      emit(log_fn, user, fmt("%.6e", now_s() - t0) + "," + std::to_string(total) + "," +
+                             fmt("%.14e", r.residual) + "," + fmt("%.14e", r.objective) + "," +
+                             fmt("%.14e", eta) + ",annealed_sinkhorn_cellsim\n");
+    if (eta == eta_final) break;
+    const double f = eta / preveta;
+    HIP_TRY(launch_axpby(st.phi, st.phi, 1.0 + f, phiprev, -f, n, ctx->stream));
+    HIP_TRY(launch_axpby(st.psi, st.psi, 1.0 + f, psiprev, -f, m, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(phiprev, st.phi, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(psiprev, st.psi, sizeof(double) * m, hipMemcpyDeviceToDevice, ctx->stream));
+    preveta = eta;
+    eta = std::fmax(q * eta, eta_final);
+  }
+  TRY(copy_out(ctx, phi_out, st.phi, n));
+  TRY(copy_out(ctx, psi_out, st.psi, m));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (out) {
+    *out = r;
+    out->iterations = total;
+    out->eta_t = eta;
+    out->elapsed_s = now_s() - t0;
+    out->pairs = ctx->pairs - pairs0;
+    out->passes = ctx->passes - passes0;
   }
   return CULAMP_OK;
 }

@@ -27,6 +27,9 @@ inline int cost_of_p(double p) {
 //   KIND_MIN : min_s C(t, s) + alpha * srcA[s]
 enum Kind : int { KIND_LSE = 0, KIND_SUM = 1, KIND_BARY = 2, KIND_MAX = 3, KIND_MIN = 4 };
 
+// Feature metrics of src/CellUtilities.jl:5 ("pearson" = correlation).
+enum Metric : int { METRIC_L1 = 0, METRIC_L2 = 1, METRIC_COSINE = 2, METRIC_CORRELATION = 3 };
+
 // Per-target epilogue applied once the split partials are merged.
 enum Epi : int {
   EPI_LSE = 0,       // out0[t] = LSE
@@ -60,11 +63,26 @@ struct PassDesc {
   double eta_mu, minv, maxv, upd_winf;
   double* th_out;
   const double* nu_in; const double* mix_src; double* nu_out; double mix_t;
+  // Feature (cell-similarity) passes: metric >= 0 selects the feature kernel.
+  // Rows are column-major (row index fastest, leading dimension = row
+  // count): divided by their sums for l1, raw for the dot metrics; rc =
+  // [n~ | mu~ | c~ | s] per row (s^-2 * squared norm, s^-1 * mean,
+  // s^-2 * centred squared norm, row sum).
+  int metric;
+  int64_t dim;
+  double fscale;
+  const double* tf64; const double* sf64;
+  const float* tf32; const float* sf32;
+  const double* trc; const double* src_rc;
+  // MFMA feature path: unit rows in bf16 (hi, and lo = remainder), row-major
+  // with width 16 * zks.
+  const __bf16* tz_hi; const __bf16* tz_lo;
+  const __bf16* sz_hi; const __bf16* sz_lo;
+  int zks;
 };
 
 struct Work {
   double* partial;     // split partials, [field][split][target]
-  unsigned* counters;  // one arrival counter per target block (self-resetting)
 };
 
 struct LaunchCfg {
@@ -87,6 +105,18 @@ hipError_t launch_pass_fp32(const PassDesc& d, int kind, const LaunchCfg& c,
                             const Work& w, hipStream_t s);
 hipError_t launch_pass_fp64(const PassDesc& d, int kind, const LaunchCfg& c,
                             const Work& w, hipStream_t s);
+// Feature passes (64 targets per workgroup; LaunchCfg.rpt is unused).
+constexpr int kFeatTargets = 64;
+LaunchCfg choose_feature_launch(int64_t nt, int64_t ns, int num_cu, int force_splits);
+hipError_t launch_feature_fp32(const PassDesc& d, int kind, const LaunchCfg& c,
+                               const Work& w, hipStream_t s);
+hipError_t launch_feature_fp64(const PassDesc& d, int kind, const LaunchCfg& c,
+                               const Work& w, hipStream_t s);
+// MFMA feature passes (128 targets per workgroup): nsplit 1 (bf16) or 3
+// (bf16 head + remainder).
+LaunchCfg choose_mfma_launch(int64_t nt, int64_t ns, int num_cu, int force_splits);
+hipError_t launch_feature_mfma(const PassDesc& d, int kind, int nsplit, const LaunchCfg& c,
+                               const Work& w, hipStream_t s);
 
 // Small device reductions used by the host loops (deterministic, one block).
 // ops[k] selects: 0 sum a, 1 sum |a - b|, 2 sum a*b, 3 max a, 4 sum a log a

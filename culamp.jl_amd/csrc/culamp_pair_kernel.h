@@ -81,21 +81,250 @@ constexpr int kind_fields() {
   return KIND == KIND_BARY ? 5 : (KIND == KIND_MAX || KIND == KIND_MIN) ? 1 : 2;
 }
 
+// Per-target accumulator state of a pass.  R is the pair arithmetic type; in
+// mixed precision `m` is an integer-valued FP32 running max in log2 units.
+template <typename R>
+struct TState {
+  R m;
+  double S, Sc, Sy0, Sy1, Sy2;
+  R acc;
+  double qd;  // exact value of the per-target exponent offset used
+};
+
+template <int PREC, typename R>
+__device__ __forceinline__ R empty_max() {
+  return PREC == 0 ? (R)-1e300 : (R)-1e30f;
+}
+
+// Epilogue of one target once its state is complete.
+template <int PREC, int KIND, typename R>
+__device__ __forceinline__ void epilogue_one(const PassDesc& d, const int64_t t, const TState<R>& st) {
+  if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
+    d.out0[t] = (double)st.acc;
+  } else if constexpr (KIND == KIND_BARY) {
+    d.out0[3 * t] = st.Sy0 / st.S;
+    d.out0[3 * t + 1] = st.Sy1 / st.S;
+    d.out0[3 * t + 2] = st.Sy2 / st.S;
+  } else if constexpr (KIND == KIND_LSE) {
+    double lse;
+    if ((double)st.m <= (double)empty_max<PREC, R>() || st.S == 0.0) lse = -INFINITY;
+    else if (PREC == 1) lse = ((double)st.m + log2(st.S) - st.qd) * kLn2;
+    else lse = (double)st.m + log(st.S);
+    if (d.epi == EPI_SK_HALF) d.out0[t] = log(d.marg[t]) - lse;
+    else d.out0[t] = lse;
+  } else {  // KIND_SUM
+    double a = st.S, c = st.Sc;
+    if (PREC == 1 && d.tgtA != nullptr) {
+      const double corr = exp2(d.gamma * d.tgtA[t] * kLog2e - st.qd);
+      a *= corr;
+      c *= corr;
+    }
+    if (d.epi == EPI_SK_RESID) {
+      d.out0[t] = d.marg[t] - a;
+      if (d.out1 != nullptr) d.out1[t] = c;
+    } else {
+      d.out0[t] = a;
+      if (d.out1 != nullptr) d.out1[t] = c;
+      if (d.epi == EPI_LAMP_COL && d.fuse_update) {
+        const double th0 = d.th0[t];
+        const double mixv = d.mix_src[t];
+        const double th = theta_update(a, d.cvec[t], d.etamu[t], th0, d.eta_mu,
+                                       d.adjust, d.minv, d.maxv, d.upd_winf);
+        d.th_out[t] = th;
+        d.nu_out[t] = (1.0 - d.mix_t) * d.nu_in[t] + d.mix_t * mixv;
+      }
+    }
+  }
+}
+
+// With one split the workgroup applies the epilogue itself; with several it
+// writes its partial state ([field][split][target]) and pass_merge, launched
+// next on the same stream, combines the splits in split order.
+template <int PREC, int KIND, int NT, typename R>
+__device__ __forceinline__ void finish_pass(const PassDesc& d, const Work& w,
+                                            const int splits, const int split,
+                                            const int64_t ntp, const int64_t (&tidx)[NT],
+                                            const TState<R> (&st)[NT]) {
+  constexpr int NF = kind_fields<KIND>();
+  if (splits > 1) {
+#pragma unroll
+    for (int r = 0; r < NT; ++r) {
+      if (tidx[r] >= ntp) continue;
+      double f[5];
+      if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) f[0] = (double)st[r].acc;
+      else if constexpr (KIND == KIND_SUM) { f[0] = st[r].S; f[1] = st[r].Sc; }
+      else { f[0] = (double)st[r].m; f[1] = st[r].S; f[2] = st[r].Sy0; f[3] = st[r].Sy1; f[4] = st[r].Sy2; }
+#pragma unroll
+      for (int q = 0; q < NF; ++q)
+        w.partial[((int64_t)q * splits + split) * ntp + tidx[r]] = f[q];
+    }
+    return;
+  }
+#pragma unroll
+  for (int r = 0; r < NT; ++r)
+    if (tidx[r] < d.nt) epilogue_one<PREC, KIND, R>(d, tidx[r], st[r]);
+}
+
+// Per-target exponent offset q_t: the target term of a SUM pass, or (mixed
+// LSE passes of the solver loops) minus the target's previous LSE so that
+// FP32 exponents stay near zero.  Returns q in natural units.
+template <int PREC, int KIND>
+__device__ __forceinline__ double target_offset(const PassDesc& d, int64_t t) {
+  double q = 0.0;
+  if (KIND == KIND_SUM && d.tgtA != nullptr) q = d.gamma * d.tgtA[t];
+  if (PREC == 1 && KIND == KIND_LSE && d.prev_offset) {
+    const double prev = d.out0[t];
+    const double lse_prev = d.epi == EPI_SK_HALF ? log(d.marg[t]) - prev : prev;
+    q = isfinite(lse_prev) ? -lse_prev : 0.0;
+  }
+  return q;
+}
+
+// Combine the split partials of each target in split order, then apply the
+// epilogue (one thread per target).
+template <int PREC, int KIND>
+__global__ __launch_bounds__(kThreads) void pass_merge(const PassDesc d, const Work w,
+                                                       const int splits, const int64_t ntp) {
+  using R = typename std::conditional<PREC == 0, double, float>::type;
+  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= d.nt) return;
+  TState<R> st;
+  st.m = empty_max<PREC, R>();
+  st.S = st.Sc = st.Sy0 = st.Sy1 = st.Sy2 = 0.0;
+  st.acc = (R)0;
+  {
+    const double q = target_offset<PREC, KIND>(d, t);
+    st.qd = PREC == 1 ? (double)(float)(q * kLog2e) : q;
+  }
+  if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
+    double a = w.partial[t];
+    for (int q = 1; q < splits; ++q) {
+      const double b = w.partial[(int64_t)q * ntp + t];
+      a = KIND == KIND_MAX ? fmax(a, b) : fmin(a, b);
+    }
+    st.acc = (R)a;
+  } else if constexpr (KIND == KIND_SUM) {
+    double a = 0.0, c = 0.0;
+    for (int q = 0; q < splits; ++q) {
+      a += w.partial[(int64_t)q * ntp + t];
+      c += w.partial[((int64_t)splits + q) * ntp + t];
+    }
+    st.S = a;
+    st.Sc = c;
+  } else {
+    double M = -INFINITY;
+    for (int q = 0; q < splits; ++q) M = fmax(M, w.partial[(int64_t)q * ntp + t]);
+    double a = 0.0, y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    for (int q = 0; q < splits; ++q) {
+      const double mq = w.partial[(int64_t)q * ntp + t];
+      double fct;
+      if (PREC == 1) fct = ldexp(1.0, (int)fmax(mq - M, -2000.0));
+      else fct = exp(mq - M);
+      a += fct * w.partial[((int64_t)splits + q) * ntp + t];
+      if (KIND == KIND_BARY) {
+        y0 += fct * w.partial[((int64_t)2 * splits + q) * ntp + t];
+        y1 += fct * w.partial[((int64_t)3 * splits + q) * ntp + t];
+        y2 += fct * w.partial[((int64_t)4 * splits + q) * ntp + t];
+      }
+    }
+    st.m = (R)M;
+    st.S = a;
+    st.Sy0 = y0; st.Sy1 = y1; st.Sy2 = y2;
+  }
+  epilogue_one<PREC, KIND, R>(d, t, st);
+}
+
+template <int PREC, int KIND>
+hipError_t launch_merge(const PassDesc& d, const LaunchCfg& c, const Work& w, int64_t ntp,
+                        hipStream_t s) {
+  if (c.splits <= 1) return hipSuccess;
+  const unsigned blocks = (unsigned)((d.nt + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL((pass_merge<PREC, KIND>), dim3(blocks), dim3(kThreads), 0, s, d, w,
+                     c.splits, ntp);
+  return hipGetLastError();
+}
+
+// Source exponent term and weight as staged in LDS (see the header comment).
+template <int PREC, bool EXPKIND, typename R>
+__device__ __forceinline__ void source_terms(const PassDesc& d, int64_t s, R* a_out, R* w_out) {
+  const double av = d.srcA != nullptr ? d.alpha * d.srcA[s] : 0.0;
+  const double bw = d.srcB != nullptr ? d.srcB[s] : 1.0;
+  if (PREC == 1 && EXPKIND) {
+    const double a2 = av * kLog2e;
+    const float hi = (float)a2;
+    const double lo = a2 - (double)hi;
+    *a_out = (R)hi;
+    *w_out = (R)(isfinite(lo) ? bw * (1.0 + lo * kLn2) : bw);
+  } else {
+    *a_out = (R)av;
+    *w_out = (R)bw;
+  }
+}
+
+// Fold NV exponents of one target into its running state.
+template <int PREC, int KIND, int NV, typename R>
+__device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], const R (&wv)[NV],
+                                           const R (&Cv)[NV], const R (&px)[NV],
+                                           const R (&py)[NV], const R (&pz)[NV]) {
+  if constexpr (KIND == KIND_LSE || KIND == KIND_BARY) {
+    R tm = v[0];
+#pragma unroll
+    for (int u = 1; u < NV; ++u) tm = tm > v[u] ? tm : v[u];
+    if constexpr (PREC == 1) {
+      const float mn = fmaxf(st.m, ceilf(tm));
+      const int sh = (int)fmaxf(st.m - mn, -2000.0f);
+      st.S *= ldexp(1.0, sh);
+      st.m = mn;
+      float loc = 0.0f;
+#pragma unroll
+      for (int u = 0; u < NV; ++u) loc = fmaf(exp2_fast(v[u] - mn), wv[u], loc);
+      st.S += (double)loc;
+    } else {
+      const double mn = fmax((double)st.m, (double)tm);
+      if (mn > (double)st.m) {
+        const double f = exp((double)st.m - mn);
+        st.S *= f;
+        if (KIND == KIND_BARY) { st.Sy0 *= f; st.Sy1 *= f; st.Sy2 *= f; }
+        st.m = (R)mn;
+      }
+      double loc = 0.0, l0 = 0.0, l1 = 0.0, l2 = 0.0;
+#pragma unroll
+      for (int u = 0; u < NV; ++u) {
+        const double e = exp((double)v[u] - mn) * (double)wv[u];
+        loc += e;
+        if (KIND == KIND_BARY) {
+          l0 = fma(e, (double)px[u], l0);
+          l1 = fma(e, (double)py[u], l1);
+          l2 = fma(e, (double)pz[u], l2);
+        }
+      }
+      st.S += loc;
+      if (KIND == KIND_BARY) { st.Sy0 += l0; st.Sy1 += l1; st.Sy2 += l2; }
+    }
+  } else if constexpr (KIND == KIND_SUM) {
+    R ls = (R)0, lc = (R)0;
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+      R e;
+      if constexpr (PREC == 1) e = exp2_fast(v[u]) * wv[u];
+      else e = exp(v[u]) * wv[u];
+      ls += e;
+      lc = fma(e, Cv[u], lc);
+    }
+    st.S += (double)ls;
+    st.Sc += (double)lc;
+  }
+}
+
 template <int PREC, int KIND, int COST, int RPT>
 __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
                                                       const Work w,
                                                       const int splits,
                                                       const int64_t split_len) {
   using R = typename std::conditional<PREC == 0, double, float>::type;
-  constexpr int NF = kind_fields<KIND>();
   constexpr bool kExpKind = (KIND == KIND_LSE || KIND == KIND_SUM || KIND == KIND_BARY);
-  constexpr bool kMaxShift = (KIND == KIND_LSE || KIND == KIND_BARY);
-  // "empty" running max: finite so that differences stay finite
-  constexpr double kEmptyD = PREC == 0 ? -1e300 : -1e30;
-  const R kEmpty = (R)kEmptyD;
 
   __shared__ R s_x[kTile], s_y[kTile], s_z[kTile], s_a[kTile], s_w[kTile];
-  __shared__ int s_last;
 
   const int tid = threadIdx.x;
   const int64_t tb = blockIdx.x;
@@ -107,8 +336,8 @@ __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
   const R pexp = (R)d.p;
 
   R tx[RPT], ty[RPT], tz[RPT], tq[RPT];
-  double qd[RPT];
   int64_t tidx[RPT];
+  TState<R> st[RPT];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
     const int64_t t = tb * kThreads * RPT + (int64_t)r * kThreads + tid;
@@ -117,31 +346,18 @@ __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
     tx[r] = ok ? (R)d.tgt[3 * t] : (R)0;
     ty[r] = ok ? (R)d.tgt[3 * t + 1] : (R)0;
     tz[r] = ok ? (R)d.tgt[3 * t + 2] : (R)0;
-    double q = 0.0;
-    if (KIND == KIND_SUM && d.tgtA != nullptr && ok) q = d.gamma * d.tgtA[t];
-    if (PREC == 1 && KIND == KIND_LSE && d.prev_offset && ok) {
-      const double prev = d.out0[t];
-      const double lse_prev = d.epi == EPI_SK_HALF ? log(d.marg[t]) - prev : prev;
-      q = isfinite(lse_prev) ? -lse_prev : 0.0;
-    }
+    const double q = ok ? target_offset<PREC, KIND>(d, t) : 0.0;
     if (PREC == 1) {
       const float q2 = (float)(q * kLog2e);
       tq[r] = (R)q2;
-      qd[r] = (double)q2;
+      st[r].qd = (double)q2;
     } else {
       tq[r] = (R)q;
-      qd[r] = q;
+      st[r].qd = q;
     }
-  }
-
-  R m[RPT];
-  double S[RPT], Sc[RPT], Sy0[RPT], Sy1[RPT], Sy2[RPT];
-  R acc[RPT];
-#pragma unroll
-  for (int r = 0; r < RPT; ++r) {
-    m[r] = kEmpty;
-    S[r] = Sc[r] = Sy0[r] = Sy1[r] = Sy2[r] = 0.0;
-    acc[r] = KIND == KIND_MIN ? (R)INFINITY : (R)-INFINITY;
+    st[r].m = empty_max<PREC, R>();
+    st[r].S = st[r].Sc = st[r].Sy0 = st[r].Sy1 = st[r].Sy2 = 0.0;
+    st[r].acc = KIND == KIND_MIN ? (R)INFINITY : (R)-INFINITY;
   }
 
   for (int64_t base = s0; base < s1; base += kTile) {
@@ -153,18 +369,7 @@ __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
       s_x[tid] = (R)d.src[3 * s];
       s_y[tid] = (R)d.src[3 * s + 1];
       s_z[tid] = (R)d.src[3 * s + 2];
-      const double av = d.srcA != nullptr ? d.alpha * d.srcA[s] : 0.0;
-      const double bw = d.srcB != nullptr ? d.srcB[s] : 1.0;
-      if (PREC == 1 && kExpKind) {
-        const double a2 = av * kLog2e;
-        const float hi = (float)a2;
-        const double lo = a2 - (double)hi;
-        s_a[tid] = (R)hi;
-        s_w[tid] = (R)(isfinite(lo) ? bw * (1.0 + lo * kLn2) : bw);
-      } else {
-        s_a[tid] = (R)av;
-        s_w[tid] = (R)bw;
-      }
+      source_terms<PREC, kExpKind>(d, s, &s_a[tid], &s_w[tid]);
     } else if (tid < cntp) {
       s_x[tid] = s_y[tid] = s_z[tid] = (R)0;
       s_a[tid] = (R)-INFINITY;
@@ -178,10 +383,10 @@ __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
           const R C = pair_cost<COST>(tx[r], ty[r], tz[r], sx, sy, sz, pexp);
-          if constexpr (KIND == KIND_MAX) acc[r] = acc[r] > C ? acc[r] : C;
+          if constexpr (KIND == KIND_MAX) st[r].acc = st[r].acc > C ? st[r].acc : C;
           else {
             const R v = C + sa;
-            acc[r] = acc[r] < v ? acc[r] : v;
+            st[r].acc = st[r].acc < v ? st[r].acc : v;
           }
         }
       }
@@ -203,171 +408,427 @@ __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
             v[r][u] = fma(k, C, sa + tq[r]);
           }
         }
-        if constexpr (kMaxShift) {
 #pragma unroll
-          for (int r = 0; r < RPT; ++r) {
-            R tm = v[r][0];
+        for (int r = 0; r < RPT; ++r)
+          accumulate<PREC, KIND, kSub, R>(st[r], v[r], wv, Cv[r], px, py, pz);
+      }
+    }
+  }
+  finish_pass<PREC, KIND, RPT, R>(d, w, splits, split, ntp, tidx, st);
+}
+
+
+// ---- feature (cell-similarity) costs ------------------------------------------
+// Cost from the accumulated statistic `acc` (sum |x~ - y~| of the rows divided
+// by their sums for l1, the raw dot x . y otherwise) and the two rows'
+// constants, with feature_cost_cuda's operations (src/CellUtilities.jl:
+// 211-282): dot = acc / (s_i s_j), then the metric.
+template <int METRIC, typename R>
+__device__ __forceinline__ R feature_cost(R acc, R ni, R mi, R ci, R si, R nj, R mj, R cj, R sj,
+                                          R dim, R scale) {
+  if constexpr (METRIC == METRIC_L1) {
+    return acc / scale;
+  } else {
+    const R dot = acc / (si * sj);
+    if constexpr (METRIC == METRIC_L2) {
+      const R v = ni + nj - (R)2 * dot;
+      return sqrt(v > (R)0 ? v : (R)0) / scale;
+    } else if constexpr (METRIC == METRIC_COSINE) {
+      const R den = sqrt(ni * nj);
+      if (!(den > (R)0)) return (R)0;
+      R sim = dot / den;
+      sim = sim < (R)-1 ? (R)-1 : (sim > (R)1 ? (R)1 : sim);
+      return ((R)1 - sim) / scale;
+    } else {
+      const R den = sqrt(ci * cj);
+      if (den <= (R)0) {
+        const R dm = mi - mj;
+        return (dm < (R)0 ? -dm : dm) <= (R)1.4901161193847656e-08 ? (R)0 : (R)1 / scale;
+      }
+      R cor = (dot - dim * mi * mj) / den;
+      cor = cor < (R)-1 ? (R)-1 : (cor > (R)1 ? (R)1 : cor);
+      return ((R)1 - cor) / scale;
+    }
+  }
+}
+
+template <int PREC, int KIND, int METRIC>
+__global__ __launch_bounds__(kThreads) void feature_pass(const PassDesc d, const Work w,
+                                                         const int splits,
+                                                         const int64_t split_len) {
+  using R = typename std::conditional<PREC == 0, double, float>::type;
+  constexpr int TT = kFeatTargets, ST = 64, KC = 16, RED = 17;
+  constexpr bool kExpKind = (KIND == KIND_LSE || KIND == KIND_SUM);
+  __shared__ R Xs[KC][TT];
+  __shared__ R Ys[KC][ST];
+  __shared__ R s_a[ST], s_w[ST], s_n[ST], s_m[ST], s_c[ST], s_s[ST];
+  __shared__ int s_valid[ST];
+  __shared__ double red0[TT][RED], red1[TT][RED];
+
+  const int tid = threadIdx.x;
+  const int ty = tid >> 4, tx = tid & 15;
+  const int64_t tb = blockIdx.x;
+  const int split = blockIdx.y;
+  const int64_t s0 = (int64_t)split * split_len;
+  const int64_t s1 = min(d.ns, s0 + split_len);
+  const int64_t ntp = (int64_t)gridDim.x * TT;
+  const R k = PREC == 0 ? (R)d.k : (R)(d.k * kLog2e);
+  const R dimR = (R)d.dim, scale = (R)d.fscale;
+  const R* X = PREC == 0 ? (const R*)d.tf64 : (const R*)d.tf32;
+  const R* Y = PREC == 0 ? (const R*)d.sf64 : (const R*)d.sf32;
+
+  R tn[4], tm[4], tcn[4], ts[4], tq[4];
+  TState<R> st[4];
 #pragma unroll
-            for (int u = 1; u < kSub; ++u) tm = tm > v[r][u] ? tm : v[r][u];
-            if constexpr (PREC == 1) {
-              const float mn = fmaxf(m[r], ceilf(tm));
-              const int sh = (int)fmaxf(m[r] - mn, -2000.0f);
-              const double f = ldexp(1.0, sh);
-              S[r] *= f;
-              if (KIND == KIND_BARY) { Sy0[r] *= f; Sy1[r] *= f; Sy2[r] *= f; }
-              m[r] = mn;
-              float loc = 0.0f;
+  for (int a = 0; a < 4; ++a) {
+    const int64_t t = tb * TT + ty * 4 + a;
+    const bool ok = t < d.nt;
+    tn[a] = ok ? (R)d.trc[t] : (R)0;
+    tm[a] = ok ? (R)d.trc[d.nt + t] : (R)0;
+    tcn[a] = ok ? (R)d.trc[2 * d.nt + t] : (R)0;
+    ts[a] = ok ? (R)d.trc[3 * d.nt + t] : (R)1;
+    const double q = ok ? target_offset<PREC, KIND>(d, t) : 0.0;
+    if (PREC == 1) {
+      const float q2 = (float)(q * kLog2e);
+      tq[a] = (R)q2;
+      st[a].qd = (double)q2;
+    } else {
+      tq[a] = (R)q;
+      st[a].qd = q;
+    }
+    st[a].m = empty_max<PREC, R>();
+    st[a].S = st[a].Sc = st[a].Sy0 = st[a].Sy1 = st[a].Sy2 = 0.0;
+    // max_logsumexp_cellsim! starts its running max at 0 (MirrorProxCellSim.jl:21)
+    st[a].acc = KIND == KIND_MIN ? (R)INFINITY : (R)0;
+  }
+
+  for (int64_t base = s0; base < s1; base += ST) {
+    const int cnt = (int)min((int64_t)ST, s1 - base);
+    R acc[4][4];
 #pragma unroll
-              for (int u = 0; u < kSub; ++u) loc = fmaf(exp2_fast(v[r][u] - mn), wv[u], loc);
-              S[r] += (double)loc;
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = (R)0;
+    for (int64_t k0 = 0; k0 < d.dim; k0 += KC) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < (KC * TT) / kThreads; ++q) {
+        const int e = tid + q * kThreads;
+        const int kk = e / TT, tt = e % TT;
+        const int64_t kd = k0 + kk;
+        const int64_t t = tb * TT + tt;
+        const int64_t s = base + tt;
+        Xs[kk][tt] = (t < d.nt && kd < d.dim) ? X[t + kd * d.nt] : (R)0;
+        Ys[kk][tt] = (tt < cnt && kd < d.dim) ? Y[s + kd * d.ns] : (R)0;
+      }
+      if (k0 == 0 && tid < ST) {
+        const int64_t s = base + tid;
+        if (tid < cnt) {
+          source_terms<PREC, kExpKind>(d, s, &s_a[tid], &s_w[tid]);
+          s_n[tid] = (R)d.src_rc[s];
+          s_m[tid] = (R)d.src_rc[d.ns + s];
+          s_c[tid] = (R)d.src_rc[2 * d.ns + s];
+          s_s[tid] = (R)d.src_rc[3 * d.ns + s];
+          s_valid[tid] = 1;
+        } else {
+          s_a[tid] = (R)-INFINITY;
+          s_w[tid] = (R)0;
+          s_n[tid] = s_m[tid] = s_c[tid] = (R)0;
+          s_s[tid] = (R)1;
+          s_valid[tid] = 0;
+        }
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = 0; kk < KC; ++kk) {
+        R xa[4], yb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) xa[a] = Xs[kk][ty * 4 + a];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) yb[b] = Ys[kk][tx * 4 + b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            if constexpr (METRIC == METRIC_L1) {
+              const R df = xa[a] - yb[b];
+              acc[a][b] += df < (R)0 ? -df : df;
+            } else if constexpr (PREC == 0) {
+              // raw_row_dot_cuda's `dot += x * y`: rounded product, then sum
+              acc[a][b] = __dadd_rn(acc[a][b], __dmul_rn(xa[a], yb[b]));
             } else {
-              const double mn = fmax((double)m[r], (double)tm);
-              if (mn > (double)m[r]) {
-                const double f = exp((double)m[r] - mn);
-                S[r] *= f;
-                if (KIND == KIND_BARY) { Sy0[r] *= f; Sy1[r] *= f; Sy2[r] *= f; }
-                m[r] = (R)mn;
-              }
-              double loc = 0.0, l0 = 0.0, l1 = 0.0, l2 = 0.0;
-#pragma unroll
-              for (int u = 0; u < kSub; ++u) {
-                const double e = exp((double)v[r][u] - mn) * (double)wv[u];
-                loc += e;
-                if (KIND == KIND_BARY) {
-                  l0 = fma(e, (double)px[u], l0);
-                  l1 = fma(e, (double)py[u], l1);
-                  l2 = fma(e, (double)pz[u], l2);
-                }
-              }
-              S[r] += loc;
-              if (KIND == KIND_BARY) { Sy0[r] += l0; Sy1[r] += l1; Sy2[r] += l2; }
+              acc[a][b] = fma(xa[a], yb[b], acc[a][b]);
             }
           }
-        } else {  // KIND_SUM
+      }
+    }
+    // costs and exponents of the 4 x 4 pairs of this thread
 #pragma unroll
-          for (int r = 0; r < RPT; ++r) {
-            R ls = (R)0, lc = (R)0;
+    for (int a = 0; a < 4; ++a) {
+      R v[4], wv[4], Cv[4], dz[4];
 #pragma unroll
-            for (int u = 0; u < kSub; ++u) {
-              R e;
-              if constexpr (PREC == 1) e = exp2_fast(v[r][u]) * wv[u];
-              else e = exp(v[r][u]) * wv[u];
-              ls += e;
-              lc = fma(e, Cv[r][u], lc);
-            }
-            S[r] += (double)ls;
-            Sc[r] += (double)lc;
+      for (int b = 0; b < 4; ++b) {
+        const int j = tx * 4 + b;
+        const R C = feature_cost<METRIC, R>(acc[a][b], tn[a], tm[a], tcn[a], ts[a], s_n[j],
+                                            s_m[j], s_c[j], s_s[j], dimR, scale);
+        const bool valid = s_valid[j] != 0;
+        Cv[b] = C;
+        wv[b] = s_w[j];
+        dz[b] = (R)0;
+        if constexpr (KIND == KIND_MAX) {
+          if (valid) st[a].acc = st[a].acc > C ? st[a].acc : C;
+        } else if constexpr (KIND == KIND_MIN) {
+          if (valid) {
+            const R vv = C + s_a[j];
+            st[a].acc = st[a].acc < vv ? st[a].acc : vv;
           }
+        } else {
+          v[b] = valid ? fma(k, C, s_a[j] + tq[a]) : (R)-INFINITY;
         }
       }
+      if constexpr (kExpKind) accumulate<PREC, KIND, 4, R>(st[a], v, wv, Cv, dz, dz, dz);
     }
   }
 
-  // ---- merge split partials (last-arriving workgroup) ----------------------
-  if (splits > 1) {
+  // ---- combine the 16 column threads of each target (fixed order) ----------
+  __syncthreads();
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-      double f[5];
-      if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) f[0] = (double)acc[r];
-      else if constexpr (KIND == KIND_SUM) { f[0] = S[r]; f[1] = Sc[r]; }
-      else { f[0] = (double)m[r]; f[1] = S[r]; f[2] = Sy0[r]; f[3] = Sy1[r]; f[4] = Sy2[r]; }
-#pragma unroll
-      for (int q = 0; q < NF; ++q)
-        w.partial[((int64_t)q * splits + split) * ntp + tidx[r]] = f[q];
-    }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-      const unsigned prev = atomicAdd(&w.counters[tb], 1u);
-      s_last = (prev == (unsigned)(splits - 1));
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    if (tid == 0) w.counters[tb] = 0u;
-#pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-      const int64_t t = tidx[r];
-      if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
-        double a = w.partial[t];
-        for (int q = 1; q < splits; ++q) {
-          const double b = w.partial[(int64_t)q * ntp + t];
-          a = KIND == KIND_MAX ? fmax(a, b) : fmin(a, b);
-        }
-        acc[r] = (R)a;
-      } else if constexpr (KIND == KIND_SUM) {
-        double a = 0.0, c = 0.0;
-        for (int q = 0; q < splits; ++q) {
-          a += w.partial[(int64_t)q * ntp + t];
-          c += w.partial[((int64_t)splits + q) * ntp + t];
-        }
-        S[r] = a;
-        Sc[r] = c;
-      } else {
-        double M = -INFINITY;
-        for (int q = 0; q < splits; ++q) M = fmax(M, w.partial[(int64_t)q * ntp + t]);
-        double a = 0.0, y0 = 0.0, y1 = 0.0, y2 = 0.0;
-        for (int q = 0; q < splits; ++q) {
-          const double mq = w.partial[(int64_t)q * ntp + t];
-          double fct;
-          if (PREC == 1) fct = ldexp(1.0, (int)fmax(mq - M, -2000.0));
-          else fct = exp(mq - M);
-          a += fct * w.partial[((int64_t)splits + q) * ntp + t];
-          if (KIND == KIND_BARY) {
-            y0 += fct * w.partial[((int64_t)2 * splits + q) * ntp + t];
-            y1 += fct * w.partial[((int64_t)3 * splits + q) * ntp + t];
-            y2 += fct * w.partial[((int64_t)4 * splits + q) * ntp + t];
-          }
-        }
-        m[r] = (R)M;
-        S[r] = a;
-        Sy0[r] = y0; Sy1[r] = y1; Sy2[r] = y2;
-      }
-    }
-  }
-
-  // ---- epilogue --------------------------------------------------------------
-#pragma unroll
-  for (int r = 0; r < RPT; ++r) {
-    const int64_t t = tidx[r];
-    if (t >= d.nt) continue;
+  for (int a = 0; a < 4; ++a) {
+    const int tt = ty * 4 + a;
     if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
-      d.out0[t] = (double)acc[r];
-    } else if constexpr (KIND == KIND_BARY) {
-      d.out0[3 * t] = Sy0[r] / S[r];
-      d.out0[3 * t + 1] = Sy1[r] / S[r];
-      d.out0[3 * t + 2] = Sy2[r] / S[r];
-    } else if constexpr (KIND == KIND_LSE) {
-      double lse;
-      if ((double)m[r] <= (double)kEmpty || S[r] == 0.0) lse = -INFINITY;
-      else if (PREC == 1) lse = ((double)m[r] + log2(S[r]) - qd[r]) * kLn2;
-      else lse = (double)m[r] + log(S[r]);
-      if (d.epi == EPI_SK_HALF) d.out0[t] = log(d.marg[t]) - lse;
-      else d.out0[t] = lse;
-    } else {  // KIND_SUM
-      double a = S[r], c = Sc[r];
-      if (PREC == 1 && d.tgtA != nullptr) {
-        const double corr = exp2(d.gamma * d.tgtA[t] * kLog2e - qd[r]);
-        a *= corr;
-        c *= corr;
+      red0[tt][tx] = (double)st[a].acc;
+    } else if constexpr (KIND == KIND_SUM) {
+      red0[tt][tx] = st[a].S;
+      red1[tt][tx] = st[a].Sc;
+    } else {
+      red0[tt][tx] = (double)st[a].m;
+      red1[tt][tx] = st[a].S;
+    }
+  }
+  __syncthreads();
+  int64_t tidx[1];
+  TState<R> fin[1];
+  tidx[0] = tid < TT ? tb * TT + tid : ntp;
+  fin[0].m = empty_max<PREC, R>();
+  fin[0].S = fin[0].Sc = fin[0].Sy0 = fin[0].Sy1 = fin[0].Sy2 = 0.0;
+  fin[0].acc = (R)0;
+  fin[0].qd = 0.0;
+  if (tid < TT) {
+    // the target's offset: the same value every column thread used
+    const int64_t t = tb * TT + tid;
+    const double q = t < d.nt ? target_offset<PREC, KIND>(d, t) : 0.0;
+    fin[0].qd = PREC == 1 ? (double)(float)(q * kLog2e) : q;
+    if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
+      double a = red0[tid][0];
+      for (int q2 = 1; q2 < 16; ++q2) a = KIND == KIND_MAX ? fmax(a, red0[tid][q2]) : fmin(a, red0[tid][q2]);
+      fin[0].acc = (R)a;
+    } else if constexpr (KIND == KIND_SUM) {
+      double a = 0.0, c = 0.0;
+      for (int q2 = 0; q2 < 16; ++q2) { a += red0[tid][q2]; c += red1[tid][q2]; }
+      fin[0].S = a;
+      fin[0].Sc = c;
+    } else {
+      double M = red0[tid][0];
+      for (int q2 = 1; q2 < 16; ++q2) M = fmax(M, red0[tid][q2]);
+      double a = 0.0;
+      for (int q2 = 0; q2 < 16; ++q2) {
+        const double f = PREC == 1 ? ldexp(1.0, (int)fmax(red0[tid][q2] - M, -2000.0))
+                                   : exp(red0[tid][q2] - M);
+        a += f * red1[tid][q2];
       }
-      if (d.epi == EPI_SK_RESID) {
-        d.out0[t] = d.marg[t] - a;
-        if (d.out1 != nullptr) d.out1[t] = c;
-      } else {
-        d.out0[t] = a;
-        if (d.out1 != nullptr) d.out1[t] = c;
-        if (d.epi == EPI_LAMP_COL && d.fuse_update) {
-          const double th0 = d.th0[t];
-          const double mixv = d.mix_src[t];
-          const double th = theta_update(a, d.cvec[t], d.etamu[t], th0, d.eta_mu,
-                                         d.adjust, d.minv, d.maxv, d.upd_winf);
-          d.th_out[t] = th;
-          d.nu_out[t] = (1.0 - d.mix_t) * d.nu_in[t] + d.mix_t * mixv;
-        }
+      fin[0].m = (R)M;
+      fin[0].S = a;
+    }
+  }
+  finish_pass<PREC, KIND, 1, R>(d, w, splits, split, ntp, tidx, fin);
+}
+
+template <int PREC, int KIND, int METRIC>
+hipError_t launch_feature_t(const PassDesc& d, const LaunchCfg& c, const Work& w, hipStream_t s) {
+  dim3 grid((unsigned)c.tblocks, (unsigned)c.splits);
+  hipLaunchKernelGGL((feature_pass<PREC, KIND, METRIC>), grid, dim3(kThreads), 0, s, d, w,
+                     c.splits, c.split_len);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_merge<PREC, KIND>(d, c, w, c.tblocks * kFeatTargets, s);
+}
+
+template <int PREC, int KIND>
+hipError_t launch_feature_metric(const PassDesc& d, const LaunchCfg& c, const Work& w, hipStream_t s) {
+  switch (d.metric) {
+    case METRIC_L1: return launch_feature_t<PREC, KIND, METRIC_L1>(d, c, w, s);
+    case METRIC_L2: return launch_feature_t<PREC, KIND, METRIC_L2>(d, c, w, s);
+    case METRIC_COSINE: return launch_feature_t<PREC, KIND, METRIC_COSINE>(d, c, w, s);
+    default: return launch_feature_t<PREC, KIND, METRIC_CORRELATION>(d, c, w, s);
+  }
+}
+
+
+// ---- MFMA feature kernel (cosine / correlation on unit rows) ------------------
+// Rows are packed as z_i = sign(s_i) x_i / |x_i| (cosine) or
+// sign(s_i) (x_i - mean_i) / |x_i - mean_i| (correlation), so that the
+// reference's similarity (CellUtilities.jl:245-282) is the dot z_i . z_j,
+// clamped to [-1, 1].  They are stored as bf16 rows (row-major, width
+// 16*KS, zero padded), optionally with a bf16 remainder row z - bf16(z) for
+// the three-product split (NSPLIT == 3).
+//
+// One wave owns 32 targets: their rows are the B operand of
+// v_mfma_f32_32x32x16_bf16 and stay in registers.  32-source tiles are
+// staged in LDS as the A operand and shared by the 4 waves.  The 32x32
+// result is laid out so that lane l holds 16 sources of target l % 32, hence
+// the log-sum-exp runs per lane; lanes l and l + 32 merge once at the end.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+template <int KIND, int NSPLIT, int KS>
+__global__ __launch_bounds__(kThreads) void mfma_feature_pass(const PassDesc d, const Work w,
+                                                              const int splits,
+                                                              const int64_t split_len) {
+  constexpr int DP = 16 * KS;          // padded row width
+  constexpr int LDW = DP + 8;          // LDS row stride (bf16), breaks bank alignment
+  constexpr int TILE = 32;
+  __shared__ __bf16 Ah[TILE * LDW];
+  __shared__ __bf16 Al[NSPLIT == 3 ? TILE * LDW : 1];
+  __shared__ float s_a[TILE], s_w[TILE];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int64_t tb = blockIdx.x;
+  const int split = blockIdx.y;
+  const int64_t s0 = (int64_t)split * split_len;
+  const int64_t s1 = min(d.ns, s0 + split_len);
+  const int64_t ntp = (int64_t)gridDim.x * 128;
+  const __bf16* Th = d.tz_hi;
+  const __bf16* Tl = d.tz_lo;
+  const __bf16* Sh = d.sz_hi;
+  const __bf16* Sl = d.sz_lo;
+  // exponent: v = k C + a_s + q_t with C = (1 - sim) / scale
+  //           = kc * sim + (a_s + k / scale) + q_t,  kc = -k / scale
+  const float kc = (float)(-d.k * kLog2e / d.fscale);
+  const float inv_scale = (float)(1.0 / d.fscale);
+
+  const int jt = lane & 31;
+  const int64_t t = tb * 128 + wave * 32 + jt;
+  const bool tok = t < d.nt;
+  bf16x8_t bh[KS], bl[NSPLIT == 3 ? KS : 1];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const int k0 = ks * 16 + 8 * (lane >> 5);
+    if (tok) {
+      bh[ks] = *(const bf16x8_t*)(Th + t * DP + k0);
+      if constexpr (NSPLIT == 3) bl[ks] = *(const bf16x8_t*)(Tl + t * DP + k0);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) bh[ks][e] = (__bf16)0.0f;
+      if constexpr (NSPLIT == 3) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bl[ks][e] = (__bf16)0.0f;
       }
     }
   }
+  TState<float> st;
+  {
+    const double q = tok ? target_offset<1, KIND>(d, t) : 0.0;
+    const float q2 = (float)(q * kLog2e);
+    st.qd = (double)q2;
+    st.m = empty_max<1, float>();
+    st.S = st.Sc = st.Sy0 = st.Sy1 = st.Sy2 = 0.0;
+    st.acc = 0.0f;
+  }
+  const float tq = (float)st.qd;
+
+  for (int64_t base = s0; base < s1; base += TILE) {
+    const int cnt = (int)min((int64_t)TILE, s1 - base);
+    __syncthreads();
+    // stage TILE source rows: TILE * DP bf16 = TILE * KS * 2 chunks of 16 B
+    for (int e = tid; e < TILE * KS * 2; e += kThreads) {
+      const int row = e / (KS * 2), c8 = (e % (KS * 2)) * 8;
+      bf16x8_t vh, vl;
+      if (row < cnt) {
+        vh = *(const bf16x8_t*)(Sh + (base + row) * DP + c8);
+        if constexpr (NSPLIT == 3) vl = *(const bf16x8_t*)(Sl + (base + row) * DP + c8);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { vh[q] = (__bf16)0.0f; vl[q] = (__bf16)0.0f; }
+      }
+      *(bf16x8_t*)(Ah + row * LDW + c8) = vh;
+      if constexpr (NSPLIT == 3) *(bf16x8_t*)(Al + row * LDW + c8) = vl;
+    }
+    if (tid < TILE) {
+      float a = -INFINITY, wv = 0.0f;
+      if (tid < cnt) {
+        // fold the constant k / scale into the FP64 source term before the split
+        const double full = (d.srcA != nullptr ? d.alpha * d.srcA[base + tid] : 0.0) + d.k / d.fscale;
+        const double a2 = full * kLog2e;
+        const float hi = (float)a2;
+        const double lo = a2 - (double)hi;
+        const double bw = d.srcB != nullptr ? d.srcB[base + tid] : 1.0;
+        a = hi;
+        wv = (float)(isfinite(lo) ? bw * (1.0 + lo * kLn2) : bw);
+      }
+      s_a[tid] = a;
+      s_w[tid] = wv;
+    }
+    __syncthreads();
+    f32x16_t acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k0 = ks * 16 + 8 * (lane >> 5);
+      const bf16x8_t ah = *(const bf16x8_t*)(Ah + jt * LDW + k0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[ks], acc, 0, 0, 0);
+      if constexpr (NSPLIT == 3) {
+        const bf16x8_t al = *(const bf16x8_t*)(Al + jt * LDW + k0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[ks], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[ks], acc, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      float v[8], wv[8], Cv[8], dz[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = h * 8 + u;
+        const int i = (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
+        float sim = acc[r];
+        sim = sim < -1.0f ? -1.0f : (sim > 1.0f ? 1.0f : sim);
+        Cv[u] = (1.0f - sim) * inv_scale;
+        wv[u] = s_w[i];
+        dz[u] = 0.0f;
+        v[u] = i < cnt ? fmaf(kc, sim, s_a[i] + tq) : -INFINITY;
+      }
+      accumulate<1, KIND, 8, float>(st, v, wv, Cv, dz, dz, dz);
+    }
+  }
+  // merge the two half-waves holding the same target
+  if constexpr (KIND == KIND_LSE) {
+    const float mo = __shfl_xor(st.m, 32);
+    const double So = __shfl_xor(st.S, 32);
+    const float mn = fmaxf(st.m, mo);
+    st.S = ldexp(st.S, (int)fmaxf(st.m - mn, -2000.0f)) + ldexp(So, (int)fmaxf(mo - mn, -2000.0f));
+    st.m = mn;
+  } else {
+    st.S += __shfl_xor(st.S, 32);
+    st.Sc += __shfl_xor(st.Sc, 32);
+  }
+  int64_t tidx[1] = {lane < 32 ? t : ntp};
+  TState<float> fin[1] = {st};
+  finish_pass<1, KIND, 1, float>(d, w, splits, split, ntp, tidx, fin);
+}
+
+template <int KIND, int NSPLIT>
+hipError_t launch_mfma_ks(const PassDesc& d, const LaunchCfg& c, const Work& w, hipStream_t s) {
+  dim3 grid((unsigned)c.tblocks, (unsigned)c.splits);
+  switch (d.zks) {
+    case 1: hipLaunchKernelGGL((mfma_feature_pass<KIND, NSPLIT, 1>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;
+    case 2: hipLaunchKernelGGL((mfma_feature_pass<KIND, NSPLIT, 2>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;
+    case 4: hipLaunchKernelGGL((mfma_feature_pass<KIND, NSPLIT, 4>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;
+    case 8: hipLaunchKernelGGL((mfma_feature_pass<KIND, NSPLIT, 8>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;
+    default: return hipErrorInvalidValue;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_merge<1, KIND>(d, c, w, c.tblocks * 128, s);
 }
 
 template <int PREC, int KIND, int COST>
@@ -379,7 +840,9 @@ hipError_t launch_rpt(const PassDesc& d, const LaunchCfg& c, const Work& w, hipS
     case 4: hipLaunchKernelGGL((pair_pass<PREC, KIND, COST, 4>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_merge<PREC, KIND>(d, c, w, c.tblocks * kThreads * c.rpt, s);
 }
 
 template <int PREC, int KIND>

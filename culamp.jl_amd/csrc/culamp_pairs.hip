@@ -122,6 +122,45 @@ hipError_t launch_pass(const PassDesc& d, int precision, int kind,
   return launch_pass_fp64(d, kind, c, w, s);
 }
 
+LaunchCfg choose_feature_launch(int64_t nt, int64_t ns, int num_cu, int force_splits) {
+  LaunchCfg c{};
+  const int64_t want = (int64_t)8 * (num_cu > 0 ? num_cu : 256);
+  c.rpt = 1;
+  c.tblocks = (nt + kFeatTargets - 1) / kFeatTargets;
+  if (c.tblocks < 1) c.tblocks = 1;
+  int64_t splits = (want + c.tblocks - 1) / c.tblocks;
+  const int64_t cap = (ns + 63) / 64;
+  if (splits > cap) splits = cap;
+  if (force_splits > 0) splits = force_splits;
+  if (splits < 1) splits = 1;
+  if (splits > 65535) splits = 65535;
+  int64_t len = (ns + splits - 1) / splits;
+  if (len < 1) len = 1;
+  c.split_len = len;
+  c.splits = ns > 0 ? (int)((ns + len - 1) / len) : 1;
+  return c;
+}
+
+LaunchCfg choose_mfma_launch(int64_t nt, int64_t ns, int num_cu, int force_splits) {
+  LaunchCfg c{};
+  const int64_t want = (int64_t)8 * (num_cu > 0 ? num_cu : 256);
+  c.rpt = 1;
+  c.tblocks = (nt + 127) / 128;
+  if (c.tblocks < 1) c.tblocks = 1;
+  int64_t splits = (want + c.tblocks - 1) / c.tblocks;
+  const int64_t cap = (ns + 63) / 64;
+  if (splits > cap) splits = cap;
+  if (force_splits > 0) splits = force_splits;
+  if (splits < 1) splits = 1;
+  if (splits > 65535) splits = 65535;
+  int64_t len = (ns + splits - 1) / splits;
+  len = (len + 31) / 32 * 32;
+  if (len < 32) len = 32;
+  c.split_len = len;
+  c.splits = ns > 0 ? (int)((ns + len - 1) / len) : 1;
+  return c;
+}
+
 hipError_t launch_reduce(const ReduceSpec& spec, double* out, hipStream_t s) {
   hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, s, spec, out);
   return hipGetLastError();

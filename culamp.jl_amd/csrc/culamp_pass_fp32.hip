@@ -8,4 +8,16 @@ hipError_t launch_pass_fp32(const PassDesc& d, int kind, const LaunchCfg& c,
   if (kind == KIND_LSE) return launch_cost<1, KIND_LSE>(d, c, w, s);
   return launch_cost<1, KIND_SUM>(d, c, w, s);
 }
+hipError_t launch_feature_fp32(const PassDesc& d, int kind, const LaunchCfg& c,
+                               const Work& w, hipStream_t s) {
+  if (kind == KIND_LSE) return launch_feature_metric<1, KIND_LSE>(d, c, w, s);
+  return launch_feature_metric<1, KIND_SUM>(d, c, w, s);
+}
+hipError_t launch_feature_mfma(const PassDesc& d, int kind, int nsplit, const LaunchCfg& c,
+                               const Work& w, hipStream_t s) {
+  if (kind == KIND_LSE) {
+    return nsplit == 3 ? launch_mfma_ks<KIND_LSE, 3>(d, c, w, s) : launch_mfma_ks<KIND_LSE, 1>(d, c, w, s);
+  }
+  return nsplit == 3 ? launch_mfma_ks<KIND_SUM, 3>(d, c, w, s) : launch_mfma_ks<KIND_SUM, 1>(d, c, w, s);
+}
 }  // namespace culamp

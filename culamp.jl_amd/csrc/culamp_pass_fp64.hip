@@ -12,4 +12,14 @@ hipError_t launch_pass_fp64(const PassDesc& d, int kind, const LaunchCfg& c,
     default: return launch_cost<0, KIND_MIN>(d, c, w, s);
   }
 }
+hipError_t launch_feature_fp64(const PassDesc& d, int kind, const LaunchCfg& c,
+                               const Work& w, hipStream_t s) {
+  switch (kind) {
+    case KIND_LSE: return launch_feature_metric<0, KIND_LSE>(d, c, w, s);
+    case KIND_SUM: return launch_feature_metric<0, KIND_SUM>(d, c, w, s);
+    case KIND_MAX: return launch_feature_metric<0, KIND_MAX>(d, c, w, s);
+    case KIND_MIN: return launch_feature_metric<0, KIND_MIN>(d, c, w, s);
+    default: return hipErrorInvalidValue;
+  }
+}
 }  // namespace culamp

@@ -8,17 +8,21 @@ AnnealedSinkhorn.jl); kernel-level functions keep the reference kernel names
 arrays, i.e. the same bytes as the reference's ``3 x n`` column-major
 matrices.
 """
-from ._lib import (FP64, MIXED, CulampError, Context, DeviceArray, device_count,
+from ._lib import (BF16, BF16X3, FP64, MIXED, CulampError, Context, DeviceArray, device_count,
                    load)
 from .args import EOTArgs, args_from_dict, read_args_json
 from .solvers import (annealed_sinkhorn_euclidean, default_context,
                       extragradient_color_transfer, extragradient_euclidean,
                       sinkhorn_color_transfer, sinkhorn_euclidean, Problem)
 from . import kernels
+from .cellsim import (FeatureProblem, annealed_sinkhorn_cellsim, cost_scale,
+                      extragradient_cellsim, normalize_cell_cost_metric, sinkhorn_cellsim)
 
 __all__ = [
-    "FP64", "MIXED", "CulampError", "Context", "DeviceArray", "device_count", "load",
+    "BF16", "BF16X3", "FP64", "MIXED", "CulampError", "Context", "DeviceArray", "device_count", "load",
     "EOTArgs", "args_from_dict", "read_args_json", "annealed_sinkhorn_euclidean",
     "default_context", "extragradient_color_transfer", "extragradient_euclidean",
-    "sinkhorn_color_transfer", "sinkhorn_euclidean", "Problem", "kernels",
+    "sinkhorn_color_transfer", "sinkhorn_euclidean", "Problem", "kernels", "FeatureProblem",
+    "annealed_sinkhorn_cellsim", "cost_scale", "extragradient_cellsim",
+    "normalize_cell_cost_metric", "sinkhorn_cellsim",
 ]

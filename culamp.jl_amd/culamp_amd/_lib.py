@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import threading
+import weakref
 
 import numpy as np
 
@@ -25,6 +26,8 @@ CULAMP_ERR_ALLOC = 6
 
 FP64 = 0
 MIXED = 1
+BF16 = 2
+BF16X3 = 3
 
 
 class CulampError(RuntimeError):
@@ -98,6 +101,20 @@ SIGNATURES = {
                                    _P, _P, _P, _P, C.POINTER(culamp_result)]),
     "culamp_annealed_sinkhorn_solve": (_I, [_P, _P, C.POINTER(culamp_args), _I64, _D, LOG_FN, _P,
                                             _P, _P, _P, _P, C.POINTER(culamp_result)]),
+    "culamp_feature_problem_create": (_I, [_P, _P, _I64, _P, _I64, _I64, _I, _D, _P, _P,
+                                           C.POINTER(_P)]),
+    "culamp_feature_lse_rows": (_I, [_P, _P, _P, _D, _D, _D, _I, _P]),
+    "culamp_feature_col_marginal_cost": (_I, [_P, _P, _P, _P, _P, _D, _D, _D, _I, _P, _P]),
+    "culamp_feature_sinkhorn_half_step": (_I, [_P, _P, _I, _P, _P, _D, _D, _I, _P]),
+    "culamp_feature_row_residual_cost": (_I, [_P, _P, _P, _P, _P, _D, _D, _I, _P, _P]),
+    "culamp_feature_max_cost": (_I, [_P, _P, _P]),
+    "culamp_feature_minplus_rows": (_I, [_P, _P, _P, _D, _P]),
+    "culamp_lamp_cellsim_solve": (_I, [_P, _P, C.POINTER(culamp_args), _I64, LOG_FN, _P, _P, _P,
+                                       _P, C.POINTER(culamp_result)]),
+    "culamp_sinkhorn_cellsim_solve": (_I, [_P, _P, C.POINTER(culamp_args), _I64, _P, _P, LOG_FN,
+                                           _P, _P, _P, C.POINTER(culamp_result)]),
+    "culamp_annealed_sinkhorn_cellsim_solve": (_I, [_P, _P, C.POINTER(culamp_args), _I64, LOG_FN,
+                                                    _P, _P, _P, C.POINTER(culamp_result)]),
 }
 
 _lib = None
@@ -151,9 +168,20 @@ class Context:
         check(self.lib.culamp_ctx_create(device, C.byref(h)))
         self.handle = h
         self.device = device
+        self._children = weakref.WeakSet()
+
+    def adopt(self, child) -> None:
+        """Register an object holding device resources of this context; it is
+        closed before the context is destroyed."""
+        self._children.add(child)
 
     def close(self) -> None:
         if self.handle:
+            for ch in list(self._children):
+                try:
+                    ch.close()
+                except Exception:
+                    pass
             self.lib.culamp_ctx_destroy(self.handle)
             self.handle = None
 
@@ -187,6 +215,7 @@ class DeviceArray:
         p = C.c_void_p()
         check(ctx.lib.culamp_malloc(ctx.handle, max(self.n, 1) * 8, C.byref(p)))
         self.ptr = p.value
+        ctx.adopt(self)
 
     @classmethod
     def from_host(cls, ctx: Context, a: np.ndarray) -> "DeviceArray":
@@ -206,9 +235,11 @@ class DeviceArray:
         return out
 
     def free(self) -> None:
-        if self.ptr:
+        if self.ptr and self.ctx.handle:
             check(self.ctx.lib.culamp_free(self.ctx.handle, self.ptr))
-            self.ptr = None
+        self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:

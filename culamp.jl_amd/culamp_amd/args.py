@@ -3,7 +3,8 @@
 Same JSON keys and defaults as the reference's ``EOTArgs`` (missing keys keep
 their defaults, unknown keys are ignored as JSON3 does for a mutable struct).
 Two optional keys are new, both defaulting to the reference's behaviour:
-``"precision"`` in {"fp64", "mixed"} (default "fp64") and ``"stop"`` in
+``"precision"`` in {"fp64", "mixed", "bf16", "bf16x3"} (default "fp64";
+the bf16 modes apply to the cosine / correlation feature costs) and ``"stop"`` in
 {"reference", "infeas"} (default "reference"; "infeas" stops LAMP on
 infeasibility < epsilon/6 alone).
 """
@@ -14,9 +15,9 @@ import json
 import math
 from dataclasses import dataclass, fields
 
-from ._lib import FP64, MIXED, culamp_args
+from ._lib import BF16, BF16X3, FP64, MIXED, culamp_args
 
-_PRECISIONS = {"fp64": FP64, "mixed": MIXED}
+_PRECISIONS = {"fp64": FP64, "mixed": MIXED, "bf16": BF16, "bf16x3": BF16X3}
 _STOPS = {"reference": 0, "infeas": 1}
 
 

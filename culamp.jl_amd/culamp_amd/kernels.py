@@ -12,6 +12,17 @@ call is stream-ordered on the context and writes its output buffer(s).
     residual_opt                        SinkhornKernel.jl:9-81
     naive_findmaxindex_ct               SinkhornKernel.jl:124-162
     naive_findmaxindex_spp_ct / _ct_t   MirrorProxKernel.jl:97-142 / 144-189
+
+Feature (cell-similarity) kernels take a ``FeatureProblem`` (its packed rows
+stay on the device) in place of the reference's data / row-statistics
+arguments:
+
+    warp_logsumexp_spp_sim_fused        MirrorProxCellSim.jl:259-342
+    residual_cellsim_c                  MirrorProxCellSim.jl:190-257
+    max_logsumexp_cellsim               MirrorProxCellSim.jl:1-41
+    warp_min_reduce_cellsim             MirrorProxCellSim.jl:43-92
+    warp_logsumexp_cellsim_fused        SinkhornCellSim.jl:153-238
+    residual_cellsim_opt                SinkhornCellSim.jl:1-72
 """
 from __future__ import annotations
 
@@ -118,3 +129,45 @@ def naive_findmaxindex_spp_ct_t(ctx: Context, output_img: DeviceArray, img1: Dev
     check(ctx.lib.culamp_barycentric_lamp_cols(ctx.handle, img1.ptr, _n(img1), img2.ptr,
                                                _n(img2), marginal1.ptr, mu.ptr, logZi.ptr,
                                                float(reg), float(winf), float(p), output_img.ptr))
+
+
+def warp_logsumexp_spp_sim_fused(ctx: Context, output: DeviceArray, prob, theta: DeviceArray,
+                                 reg: float, st: float, winf: float,
+                                 precision: int = FP64) -> None:
+    check(ctx.lib.culamp_feature_lse_rows(ctx.handle, prob.handle, theta.ptr, float(reg),
+                                          float(st), float(winf), precision, output.ptr))
+
+
+def residual_cellsim_c(ctx: Context, output: DeviceArray, cost_output: DeviceArray | None, prob,
+                       marginal1: DeviceArray, theta: DeviceArray, logZi: DeviceArray,
+                       reg: float, st: float, winf: float, precision: int = FP64) -> None:
+    check(ctx.lib.culamp_feature_col_marginal_cost(ctx.handle, prob.handle, marginal1.ptr,
+                                                   theta.ptr, logZi.ptr, float(reg), float(st),
+                                                   float(winf), precision, output.ptr,
+                                                   dptr(cost_output)))
+
+
+def max_logsumexp_cellsim(ctx: Context, output: DeviceArray, prob) -> None:
+    check(ctx.lib.culamp_feature_max_cost(ctx.handle, prob.handle, output.ptr))
+
+
+def warp_min_reduce_cellsim(ctx: Context, output: DeviceArray, prob, theta: DeviceArray,
+                            winf: float) -> None:
+    check(ctx.lib.culamp_feature_minplus_rows(ctx.handle, prob.handle, theta.ptr, float(winf),
+                                              output.ptr))
+
+
+def warp_logsumexp_cellsim_fused(ctx: Context, output: DeviceArray, prob, rows: bool,
+                                 marginal: DeviceArray, theta: DeviceArray, reg: float,
+                                 winf: float, precision: int = FP64) -> None:
+    check(ctx.lib.culamp_feature_sinkhorn_half_step(ctx.handle, prob.handle, 1 if rows else 0,
+                                                    marginal.ptr, theta.ptr, float(reg),
+                                                    float(winf), precision, output.ptr))
+
+
+def residual_cellsim_opt(ctx: Context, output: DeviceArray, cost_output: DeviceArray | None,
+                         prob, marginal: DeviceArray, phi: DeviceArray, psi: DeviceArray,
+                         reg: float, winf: float, precision: int = FP64) -> None:
+    check(ctx.lib.culamp_feature_row_residual_cost(ctx.handle, prob.handle, marginal.ptr,
+                                                   phi.ptr, psi.ptr, float(reg), float(winf),
+                                                   precision, output.ptr, dptr(cost_output)))

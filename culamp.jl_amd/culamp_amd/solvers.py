@@ -65,11 +65,13 @@ class Problem:
         check(load().culamp_problem_create(self.ctx.handle, ptr(self.x), self.n, ptr(self.y),
                                            self.m, ptr(self.r), ptr(self.c), C.byref(h)))
         self.handle = h
+        self.ctx.adopt(self)
 
     def close(self) -> None:
-        if getattr(self, "handle", None):
+        # the context destroys its problems itself once it is closed
+        if getattr(self, "handle", None) and self.ctx.handle:
             load().culamp_problem_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

@@ -50,8 +50,20 @@ enum {
   CULAMP_ERR_ALLOC = 6
 };
 
-enum { CULAMP_FP64 = 0, CULAMP_MIXED = 1 };
+/* precision: CULAMP_FP64 and CULAMP_MIXED apply to every pass.  For the
+ * cosine and correlation feature costs, CULAMP_BF16 evaluates the row dots
+ * as one bf16 MFMA product with FP32 accumulation and CULAMP_BF16X3 as the
+ * three-product split (bf16 head and remainder); other passes treat both as
+ * CULAMP_MIXED. */
+enum { CULAMP_FP64 = 0, CULAMP_MIXED = 1, CULAMP_BF16 = 2, CULAMP_BF16X3 = 3 };
 enum { CULAMP_STOP_REFERENCE = 0, CULAMP_STOP_INFEAS = 1 };
+/* Feature metrics of src/CellUtilities.jl:5 ("pearson" = correlation). */
+enum {
+  CULAMP_METRIC_L1 = 0,
+  CULAMP_METRIC_L2 = 1,
+  CULAMP_METRIC_COSINE = 2,
+  CULAMP_METRIC_CORRELATION = 3
+};
 
 typedef struct culamp_ctx culamp_ctx;
 typedef struct culamp_problem culamp_problem;
@@ -192,6 +204,43 @@ int culamp_problem_create(culamp_ctx *ctx, const double *x, int64_t n,
                           const double *y, int64_t m, const double *r,
                           const double *c, culamp_problem **out);
 int culamp_problem_destroy(culamp_problem *prob);
+/* Cell-similarity (feature-cost) problem: rows of x (n x d) and y (m x d),
+ * both COLUMN-major like the reference's `data::Matrix` (x[i + k*n]); y may
+ * be NULL or equal to x for the reference's self-cost (data1 == data2).  The
+ * cost is feature_cost_cuda (src/CellUtilities.jl:187-301) divided by
+ * `scale` (_cost_scale, 78-84: l1 2, l2 sqrt 2, cosine 2, correlation 2). */
+int culamp_feature_problem_create(culamp_ctx *ctx, const double *x, int64_t n,
+                                  const double *y, int64_t m, int64_t d,
+                                  int metric, double scale, const double *r,
+                                  const double *c, culamp_problem **out);
+
+/* ---- feature kernel level (device vectors on a feature problem) ---------- */
+/* warp_logsumexp_spp_sim_fused! (src/solvers/MirrorProxCellSim.jl:259-342) */
+int culamp_feature_lse_rows(culamp_ctx *ctx, culamp_problem *prob,
+                            const double *theta, double reg, double st,
+                            double winf, int precision, double *out);
+/* residual_cellsim_c! (MirrorProxCellSim.jl:190-257) */
+int culamp_feature_col_marginal_cost(culamp_ctx *ctx, culamp_problem *prob,
+                                     const double *r, const double *theta,
+                                     const double *logZ, double reg, double st,
+                                     double winf, int precision, double *res,
+                                     double *cost);
+/* warp_logsumexp_cellsim_fused! (src/solvers/SinkhornCellSim.jl:153-238);
+ * rows != 0: out (n) from the column potential pot (m), else the reverse. */
+int culamp_feature_sinkhorn_half_step(culamp_ctx *ctx, culamp_problem *prob,
+                                      int rows, const double *marg,
+                                      const double *pot, double reg,
+                                      double winf, int precision, double *out);
+/* residual_cellsim_opt! (SinkhornCellSim.jl:1-72) */
+int culamp_feature_row_residual_cost(culamp_ctx *ctx, culamp_problem *prob,
+                                     const double *marg, const double *phi,
+                                     const double *psi, double reg, double winf,
+                                     int precision, double *res, double *cost);
+/* max_logsumexp_cellsim! (MirrorProxCellSim.jl:1-41): max(0, max_j C_ij) */
+int culamp_feature_max_cost(culamp_ctx *ctx, culamp_problem *prob, double *out);
+/* warp_min_reduce_cellsim! (MirrorProxCellSim.jl:43-92) */
+int culamp_feature_minplus_rows(culamp_ctx *ctx, culamp_problem *prob,
+                                const double *theta, double winf, double *out);
 
 /* ---- solver level --------------------------------------------------------- */
 /* extragradient_color_transfer (src/solvers/MirrorProxKernel.jl:1019-1132):
@@ -218,6 +267,27 @@ int culamp_annealed_sinkhorn_solve(culamp_ctx *ctx, culamp_problem *prob,
                                    double p, culamp_log_fn log_fn, void *user,
                                    double *phi, double *psi, double *img1,
                                    double *img2, culamp_result *result);
+
+/* extragradient_cellsim (src/solvers/MirrorProxCellSim.jl:94-188) on a
+ * feature problem; nu, psi: m; phi: n (any may be NULL). */
+int culamp_lamp_cellsim_solve(culamp_ctx *ctx, culamp_problem *prob,
+                              const culamp_args *args, int64_t frequency,
+                              culamp_log_fn log_fn, void *user, double *nu,
+                              double *phi, double *psi, culamp_result *result);
+/* sinkhorn_cellsim (src/solvers/SinkhornCellSim.jl:240-310) */
+int culamp_sinkhorn_cellsim_solve(culamp_ctx *ctx, culamp_problem *prob,
+                                  const culamp_args *args, int64_t frequency,
+                                  const double *phi0, const double *psi0,
+                                  culamp_log_fn log_fn, void *user, double *phi,
+                                  double *psi, culamp_result *result);
+/* annealed_sinkhorn_cellsim (src/solvers/AnnealedSinkhornCellSim.jl:1-63) */
+int culamp_annealed_sinkhorn_cellsim_solve(culamp_ctx *ctx,
+                                           culamp_problem *prob,
+                                           const culamp_args *args,
+                                           int64_t frequency,
+                                           culamp_log_fn log_fn, void *user,
+                                           double *phi, double *psi,
+                                           culamp_result *result);
 
 #ifdef __cplusplus
 }

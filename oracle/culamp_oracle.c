@@ -707,3 +707,463 @@ void or_grid_cost_matrix(int64_t height, int64_t width, double p, double *W) {
       W[i * N + j] = p < 10 ? pow(a, p) + pow(b, p) : fmax(a, b);
     }
 }
+
+/* ===================== feature (cell-similarity) costs ===================== */
+
+/* get_flat_metrics (src/CellUtilities.jl:304-311).  x is n x d column-major. */
+void or_feature_stats(const double *x, int64_t n, int64_t d, double *sums,
+                      double *sqnorms, double *means, double *csqnorms) {
+  for (int64_t i = 0; i < n; ++i) {
+    double s = 0.0, q = 0.0;
+    for (int64_t k = 0; k < d; ++k) {
+      double v = x[i + k * n];
+      s += v;
+      q += v * v;
+    }
+    double mu = s / (double)d, cs = 0.0;
+    for (int64_t k = 0; k < d; ++k) {
+      double v = x[i + k * n] - mu;
+      cs += v * v;
+    }
+    sums[i] = s;
+    sqnorms[i] = q;
+    means[i] = mu;
+    csqnorms[i] = cs;
+  }
+}
+
+static double clamp1(double v) { return v < -1.0 ? -1.0 : (v > 1.0 ? 1.0 : v); }
+
+/* feature_cost_cuda and the four metric functions (src/CellUtilities.jl:
+ * 187-301).  xs / ys hold [sums | sqnorms | means | csqnorms] (4 x n). */
+double or_feature_cost(int metric, const double *x, int64_t n, int64_t i,
+                       const double *xs, const double *y, int64_t m, int64_t j,
+                       const double *ys, int64_t d, double scale) {
+  const double si = xs[i], sj = ys[j];
+  if (metric == OR_L1) { /* 211-226 */
+    double dist = 0.0;
+    for (int64_t k = 0; k < d; ++k) dist += fabs(x[i + k * n] / si - y[j + k * m] / sj);
+    return dist / scale;
+  }
+  double raw = 0.0;
+  for (int64_t k = 0; k < d; ++k) raw += x[i + k * n] * y[j + k * m];
+  const double dot = raw / (si * sj);
+  if (metric == OR_L2) { /* 230-243 */
+    double ni = xs[n + i] / (si * si), nj = ys[m + j] / (sj * sj);
+    return sqrt(fmax(ni + nj - 2.0 * dot, 0.0)) / scale;
+  }
+  if (metric == OR_COSINE) { /* 245-259 */
+    double denom = sqrt((xs[n + i] / (si * si)) * (ys[m + j] / (sj * sj)));
+    if (!(denom > 0.0)) return 0.0;
+    return (1.0 - clamp1(dot / denom)) / scale;
+  }
+  if (metric == OR_CORRELATION) { /* 261-282 */
+    double mi = xs[2 * n + i] / si, mj = ys[2 * m + j] / sj;
+    double ci = xs[3 * n + i] / (si * si), cj = ys[3 * m + j] / (sj * sj);
+    double denom = sqrt(ci * cj);
+    if (denom <= 0.0) return fabs(mi - mj) <= sqrt(2.220446049250313e-16) ? 0.0 : 1.0 / scale;
+    return (1.0 - clamp1((dot - (double)d * mi * mj) / denom)) / scale;
+  }
+  return INFINITY; /* 300 */
+}
+
+typedef struct {
+  const double *x, *y, *xs, *ys;
+  int64_t n, m, d;
+  int metric;
+  double scale;
+} fctx;
+
+static double fcost(const fctx *f, int64_t i, int64_t j) {
+  return or_feature_cost(f->metric, f->x, f->n, i, f->xs, f->y, f->m, j, f->ys, f->d, f->scale);
+}
+
+static double *stats_alloc(const double *x, int64_t n, int64_t d) {
+  double *s = malloc(sizeof(double) * 4 * (size_t)n);
+  or_feature_stats(x, n, d, s, s + n, s + 2 * n, s + 3 * n);
+  return s;
+}
+
+void or_feature_cost_matrix(int metric, const double *x, int64_t n,
+                            const double *y, int64_t m, int64_t d, double scale,
+                            double *C) {
+  double *xs = stats_alloc(x, n, d), *ys = stats_alloc(y, m, d);
+  fctx f = {x, y, xs, ys, n, m, d, metric, scale};
+  OMP_FOR
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = 0; j < m; ++j) C[i * m + j] = fcost(&f, i, j);
+  free(xs);
+  free(ys);
+}
+
+/* Dense-cost helpers over a precomputed C (n x m row-major); the feature
+ * kernels evaluate the same C_ij per pair. */
+static void d_lse_rows(const double *C, int64_t n, int64_t m, const double *theta,
+                       double reg, double st, double winf, double *out) {
+  const double c1 = st / (2.0 * winf), invreg = 1.0 / reg;
+  OMP_FOR
+  for (int64_t i = 0; i < n; ++i) {
+    double mx = -INFINITY;
+    for (int64_t j = 0; j < m; ++j) {
+      double v = -(C[i * m + j] * c1 + theta[j]) * invreg;
+      if (v > mx) mx = v;
+    }
+    double s = 0.0;
+    if (isfinite(mx))
+      for (int64_t j = 0; j < m; ++j) s += exp(-(C[i * m + j] * c1 + theta[j]) * invreg - mx);
+    out[i] = lse_finish(mx, s);
+  }
+}
+
+static void d_colmarg(const double *C, int64_t n, int64_t m, const double *r,
+                      const double *theta, const double *logZ, double reg,
+                      double st, double winf, double *res, double *cost) {
+  const double c1 = st / (2.0 * winf), invreg = 1.0 / reg;
+  OMP_FOR
+  for (int64_t j = 0; j < m; ++j) {
+    double a = 0.0, ca = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      double c = C[i * m + j];
+      double w = exp((c * c1 + theta[j]) * (-invreg) - logZ[i]) * r[i];
+      a += w;
+      ca += w * c;
+    }
+    res[j] = a;
+    cost[j] = ca;
+  }
+}
+
+static void d_minplus(const double *C, int64_t n, int64_t m, const double *theta,
+                      double winf, double *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    double mn = INFINITY;
+    for (int64_t j = 0; j < m; ++j) {
+      double v = theta[j] * (2.0 * winf) + C[i * m + j];
+      if (v < mn) mn = v;
+    }
+    out[i] = mn;
+  }
+}
+
+/* Sinkhorn half-steps on dense C: rows (phi) and columns (psi). */
+static void d_sk_rows(const double *C, int64_t n, int64_t m, const double *marg,
+                      const double *psi, double reg, double winf, double *out) {
+  const double k = -1.0 / (reg * winf);
+  OMP_FOR
+  for (int64_t i = 0; i < n; ++i) {
+    double mx = -INFINITY;
+    for (int64_t j = 0; j < m; ++j) {
+      double v = C[i * m + j] * k + psi[j];
+      if (v > mx) mx = v;
+    }
+    double s = 0.0;
+    if (isfinite(mx))
+      for (int64_t j = 0; j < m; ++j) s += exp(C[i * m + j] * k + psi[j] - mx);
+    out[i] = log(marg[i]) - lse_finish(mx, s);
+  }
+}
+
+static void d_sk_cols(const double *C, int64_t n, int64_t m, const double *marg,
+                      const double *phi, double reg, double winf, double *out) {
+  const double k = -1.0 / (reg * winf);
+  OMP_FOR
+  for (int64_t j = 0; j < m; ++j) {
+    double mx = -INFINITY;
+    for (int64_t i = 0; i < n; ++i) {
+      double v = C[i * m + j] * k + phi[i];
+      if (v > mx) mx = v;
+    }
+    double s = 0.0;
+    if (isfinite(mx))
+      for (int64_t i = 0; i < n; ++i) s += exp(C[i * m + j] * k + phi[i] - mx);
+    out[j] = log(marg[j]) - lse_finish(mx, s);
+  }
+}
+
+static void d_sk_resid(const double *C, int64_t n, int64_t m, const double *marg,
+                       const double *phi, const double *psi, double reg,
+                       double winf, double *res, double *cost) {
+  const double k = -1.0 / (reg * winf);
+  OMP_FOR
+  for (int64_t i = 0; i < n; ++i) {
+    double a = 0.0, ca = 0.0;
+    for (int64_t j = 0; j < m; ++j) {
+      double c = C[i * m + j];
+      double w = exp((c * k + phi[i]) + psi[j]);
+      a += w;
+      ca += w * c;
+    }
+    res[i] = marg[i] - a;
+    cost[i] = ca;
+  }
+}
+
+/* max_logsumexp_cellsim! (MirrorProxCellSim.jl:1-41): running max starts at 0 */
+static double d_winf(const double *C, int64_t n, int64_t m) {
+  double w = -INFINITY;
+  for (int64_t i = 0; i < n; ++i) {
+    double mx = 0.0;
+    for (int64_t j = 0; j < m; ++j) mx = fmax(mx, C[i * m + j]);
+    w = fmax(w, mx);
+  }
+  return w;
+}
+
+/* extragradient_cellsim (src/solvers/MirrorProxCellSim.jl:94-188) */
+int or_lamp_cellsim_solve(const double *data, int64_t n, int64_t d,
+                          double scale, int metric, const double *r,
+                          const double *c, const or_args *args,
+                          int64_t frequency, or_log_fn logfn, void *user,
+                          double *nu_out, double *phi_out, double *psi_out,
+                          or_result *out) {
+  if (n <= 0 || d <= 0 || frequency <= 0) return 1;
+  const int64_t m = n;
+  double *C = malloc(sizeof(double) * (size_t)n * (size_t)m);
+  or_feature_cost_matrix(metric, data, n, data, m, d, scale, C);
+  double *theta = calloc((size_t)m, sizeof(double)), *nu = calloc((size_t)m, sizeof(double));
+  double *thetab = calloc((size_t)m, sizeof(double)), *nub = calloc((size_t)m, sizeof(double));
+  double *gam = calloc((size_t)m, sizeof(double)), *res = calloc((size_t)m, sizeof(double));
+  double *cost = calloc((size_t)m, sizeof(double)), *etamu = calloc((size_t)m, sizeof(double));
+  double *sumvals = calloc((size_t)n, sizeof(double));
+  const double winf = d_winf(C, n, m); /* 115-117 */
+  const double eta = args->eta_p;       /* 118: not divided by 2 W */
+  for (int64_t j = 0; j < m; ++j) etamu[j] = (c[j] + args->alpha / (double)n) / args->tau_mu;
+  const double t0 = now_s();
+  const double hr = eta * neg_entropy_sum(r, n);
+  emit(logfn, user, "time(s),iter,infeas,ot_objective,primal,dual,solver\n");
+  double st = 1.0, etat = INFINITY;
+  const double minv = tanh(-args->B / 2.0), maxv = tanh(args->B / 2.0);
+  double primal = NAN, dual = NAN, resid = NAN, objective = NAN;
+  int converged = 0;
+  int64_t steps = 0;
+  for (int64_t it = 1; it <= args->itermax; ++it) {
+    double elapsed = now_s() - t0;
+    if (elapsed > args->tmax) break;
+    if ((args->verbose || args->stop == 1) && (it - 1) % frequency == 0) { /* 144-166 */
+      d_lse_rows(C, n, m, nu, etat, st, winf, sumvals);
+      d_colmarg(C, n, m, r, nu, sumvals, etat, st, winf, res, cost);
+      resid = 0.0;
+      objective = 0.0;
+      for (int64_t j = 0; j < m; ++j) {
+        resid += fabs(res[j] - c[j]);
+        objective += cost[j];
+      }
+      if (eta > 0) {
+        primal = objective * (1.0 - eta / etat) - eta * dotv(r, sumvals, n) -
+                 2.0 * winf * eta / etat * dotv(c, nu, m) + hr + 2.0 * winf * resid;
+        d_lse_rows(C, n, m, theta, eta, 1.0, winf, sumvals);
+        dual = -eta * dotv(r, sumvals, n) - 2.0 * winf * dotv(c, theta, m) + hr;
+      } else {
+        primal = objective + 2.0 * winf * resid;
+        d_minplus(C, n, m, gam, winf, sumvals);
+        dual = dotv(r, sumvals, n) - 2.0 * winf * dotv(c, gam, m);
+      }
+      if (args->verbose) {
+        char b0[64], b1[64], b2[64], b3[64], b4[64], line[512];
+        fmt_e(b0, 64, "%.6e", elapsed);
+        fmt_e(b1, 64, "%.14e", resid);
+        fmt_e(b2, 64, "%.14e", objective);
+        fmt_e(b3, 64, "%.14e", primal);
+        fmt_e(b4, 64, "%.14e", dual);
+        snprintf(line, sizeof line, "%s,%lld,%s,%s,%s,%s,lamp_kernel\n", b0, (long long)it,
+                 b1, b2, b3, b4);
+        emit(logfn, user, line);
+      }
+      const int met = args->stop == 1 ? resid < args->epsilon / 6.0
+                                       : (primal - dual < args->epsilon / 6.0 && resid < args->epsilon / 6.0);
+      if (met) {
+        converged = 1;
+        break;
+      }
+    }
+    /* 168-180 */
+    d_lse_rows(C, n, m, nu, etat, st, winf, sumvals);
+    d_colmarg(C, n, m, r, nu, sumvals, etat, st, winf, res, cost);
+    or_theta_update(thetab, theta, res, c, etamu, m, args->eta_mu, 0, minv, maxv, 1.0);
+    etat = 1.0 / (1.0 + (1.0 / etat) * (1.0 - eta));
+    for (int64_t j = 0; j < m; ++j) nub[j] = (1.0 - etat) * nu[j] + etat * theta[j];
+    st = (1.0 - etat) * st + etat;
+    d_lse_rows(C, n, m, nub, etat, st, winf, sumvals);
+    d_colmarg(C, n, m, r, nub, sumvals, etat, st, winf, res, cost);
+    or_theta_update(theta, theta, res, c, etamu, m, args->eta_mu, 1, minv, maxv, 1.0);
+    for (int64_t j = 0; j < m; ++j) {
+      nu[j] = (1.0 - etat) * nu[j] + etat * thetab[j];
+      gam[j] = (1.0 - etat) * gam[j] + etat * theta[j];
+    }
+    steps += 1;
+  }
+  /* 184-187: phi uses whatever sumvals holds last */
+  double objf = 0.0;
+  for (int64_t j = 0; j < m; ++j) objf += cost[j];
+  if (psi_out)
+    for (int64_t j = 0; j < m; ++j) psi_out[j] = -2.0 * winf * nu[j] / args->eta_p;
+  if (phi_out)
+    for (int64_t i = 0; i < n; ++i) phi_out[i] = log(r[i]) - sumvals[i];
+  if (nu_out) memcpy(nu_out, nu, sizeof(double) * (size_t)m);
+  if (out) {
+    out->iterations = steps;
+    out->objective = objf;
+    out->residual = resid;
+    out->primal = primal;
+    out->dual = dual;
+    out->winf = winf;
+    out->eta_t = etat;
+    out->converged = converged;
+  }
+  free(C); free(theta); free(nu); free(thetab); free(nub); free(gam); free(res);
+  free(cost); free(etamu); free(sumvals);
+  return 0;
+}
+
+/* sinkhorn_cellsim (src/solvers/SinkhornCellSim.jl:240-310) with the cost
+ * C_ij between row i of data1 and row j of data2 (equal to the reference's
+ * per-call cost when data1 == data2, its only call pattern). */
+static int sk_cellsim_core(const double *C, int64_t n, int64_t m, const double *r,
+                           const double *c, const or_args *args, int64_t frequency,
+                           double *phi, double *psi, double *res, double *cost,
+                           or_log_fn logfn, void *user, or_result *out) {
+  const double t0 = now_s();
+  const double winf = d_winf(C, n, m); /* 269-270 */
+  const double eta = args->eta_p;
+  int64_t num_iter = 0;
+  int converged = 0;
+  if (args->verbose) emit(logfn, user, "time(s),iter,infeas,ot_objective,dual\n");
+  for (int64_t it = 1; it <= args->itermax; ++it) {
+    double elapsed = now_s() - t0;
+    if (elapsed > args->tmax) break;
+    d_sk_rows(C, n, m, r, psi, eta, winf, phi);
+    d_sk_cols(C, n, m, c, phi, eta, winf, psi);
+    if ((it - 1) % frequency == 0) {
+      d_sk_resid(C, n, m, r, phi, psi, eta, winf, res, cost);
+      double rr = 0.0;
+      for (int64_t i = 0; i < n; ++i) rr += fabs(res[i]);
+      if (args->verbose) {
+        double ot = 0.0;
+        for (int64_t i = 0; i < n; ++i) ot += cost[i];
+        char b0[64], b1[64], b2[64], b3[64], line[512];
+        fmt_e(b0, 64, "%.6e", elapsed);
+        fmt_e(b1, 64, "%.14e", rr);
+        fmt_e(b2, 64, "%.14e", ot);
+        fmt_e(b3, 64, "%.14e", dotv(psi, c, m) + dotv(phi, r, n));
+        snprintf(line, sizeof line, "%s,%lld,%s,%s,%s,sinkhorn_cellsim\n", b0, (long long)it,
+                 b1, b2, b3);
+        emit(logfn, user, line);
+      }
+      if (rr <= args->epsilon / 6.0) {
+        converged = 1;
+        break;
+      }
+    }
+    num_iter += 1;
+  }
+  d_sk_resid(C, n, m, r, phi, psi, eta, winf, res, cost);
+  double rv = 0.0, ob = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    rv += fabs(res[i]);
+    ob += cost[i];
+  }
+  if (out) {
+    out->iterations = num_iter;
+    out->objective = ob;
+    out->residual = rv;
+    out->primal = NAN;
+    out->dual = dotv(psi, c, m) + dotv(phi, r, n);
+    out->winf = winf;
+    out->eta_t = eta;
+    out->converged = converged;
+  }
+  return 0;
+}
+
+int or_sinkhorn_cellsim_solve(const double *data1, int64_t n,
+                              const double *data2, int64_t m, int64_t d,
+                              double scale, int metric, const double *r,
+                              const double *c, const or_args *args,
+                              int64_t frequency, const double *phi0,
+                              const double *psi0, or_log_fn logfn, void *user,
+                              double *phi_out, double *psi_out, or_result *out) {
+  if (n <= 0 || m <= 0 || d <= 0 || frequency <= 0) return 1;
+  double *C = malloc(sizeof(double) * (size_t)n * (size_t)m);
+  or_feature_cost_matrix(metric, data1, n, data2, m, d, scale, C);
+  double *phi = calloc((size_t)n, sizeof(double)), *psi = calloc((size_t)m, sizeof(double));
+  double *res = calloc((size_t)n, sizeof(double)), *cost = calloc((size_t)n, sizeof(double));
+  if (phi0) memcpy(phi, phi0, sizeof(double) * (size_t)n);
+  if (psi0) memcpy(psi, psi0, sizeof(double) * (size_t)m);
+  sk_cellsim_core(C, n, m, r, c, args, frequency, phi, psi, res, cost, logfn, user, out);
+  if (phi_out) memcpy(phi_out, phi, sizeof(double) * (size_t)n);
+  if (psi_out) memcpy(psi_out, psi, sizeof(double) * (size_t)m);
+  free(C); free(phi); free(psi); free(res); free(cost);
+  return 0;
+}
+
+/* annealed_sinkhorn_cellsim (src/solvers/AnnealedSinkhornCellSim.jl:1-63):
+ * no tmax test between stages, unlike the point-cloud version. */
+int or_annealed_sinkhorn_cellsim_solve(const double *data1, int64_t n,
+                                       const double *data2, int64_t m,
+                                       int64_t d, double scale, int metric,
+                                       const double *r, const double *c,
+                                       const or_args *args_in, int64_t frequency,
+                                       or_log_fn logfn, void *user, double *phi_out,
+                                       double *psi_out, or_result *out) {
+  if (n <= 0 || m <= 0 || d <= 0 || frequency <= 0) return 1;
+  double *C = malloc(sizeof(double) * (size_t)n * (size_t)m);
+  or_feature_cost_matrix(metric, data1, n, data2, m, d, scale, C);
+  or_args args = *args_in;
+  const double eta_final = args.epsilon;
+  double eta = fmax(args.eta_p, eta_final);
+  const double t0 = now_s();
+  double *phi = malloc(sizeof(double) * (size_t)n), *psi = malloc(sizeof(double) * (size_t)m);
+  double *phiprev = malloc(sizeof(double) * (size_t)n), *psiprev = malloc(sizeof(double) * (size_t)m);
+  double *res = calloc((size_t)n, sizeof(double)), *cost = calloc((size_t)n, sizeof(double));
+  for (int64_t i = 0; i < n; ++i) phi[i] = phiprev[i] = log(r[i]);
+  for (int64_t j = 0; j < m; ++j) psi[j] = psiprev[j] = log(c[j]);
+  const double q = args.anneal_mult;
+  double preveta = 1.0, objective = NAN, residual = NAN;
+  const int verbose = args.verbose;
+  args.verbose = 0;
+  int64_t total = 0;
+  if (verbose) emit(logfn, user, "time(s),iter,infeas,ot_objective,eta,solver\n");
+  or_result rs;
+  for (;;) {
+    args.eta_p = eta;
+    /* phi, psi hold the warm start phi_ws, psi_ws and are updated in place */
+    sk_cellsim_core(C, n, m, r, c, &args, frequency, phi, psi, res, cost, NULL, NULL, &rs);
+    total += rs.iterations;
+    objective = rs.objective;
+    residual = rs.residual;
+    if (verbose) {
+      char b0[64], b1[64], b2[64], b3[64], line[512];
+      fmt_e(b0, 64, "%.6e", now_s() - t0);
+      fmt_e(b1, 64, "%.14e", residual);
+      fmt_e(b2, 64, "%.14e", objective);
+      fmt_e(b3, 64, "%.14e", eta);
+      snprintf(line, sizeof line, "%s,%lld,%s,%s,%s,annealed_sinkhorn_cellsim\n", b0,
+               (long long)total, b1, b2, b3);
+      emit(logfn, user, line);
+    }
+    if (eta == eta_final) break;
+    /* 49-52: sinkhorn_cellsim updates the warm-start arrays in place
+     * (SinkhornCellSim.jl:257-259 binds phi = phi0), so after
+     * `phi_ws .= phi + f (phi - phi_prev)` the assignment `phi_prev .= phi`
+     * copies the extrapolated warm start, not the stage solution. */
+    for (int64_t i = 0; i < n; ++i) {
+      phi[i] = phi[i] + (eta / preveta) * (phi[i] - phiprev[i]);
+      phiprev[i] = phi[i];
+    }
+    for (int64_t j = 0; j < m; ++j) {
+      psi[j] = psi[j] + (eta / preveta) * (psi[j] - psiprev[j]);
+      psiprev[j] = psi[j];
+    }
+    preveta = eta;
+    eta = fmax(q * eta, eta_final);
+  }
+  if (phi_out) memcpy(phi_out, phi, sizeof(double) * (size_t)n);
+  if (psi_out) memcpy(psi_out, psi, sizeof(double) * (size_t)m);
+  if (out) {
+    *out = rs;
+    out->iterations = total;
+    out->eta_t = eta;
+  }
+  free(C); free(phi); free(psi); free(phiprev); free(psiprev); free(res); free(cost);
+  return 0;
+}

@@ -133,6 +133,55 @@ int or_sinkhorn_log_dense(const double *r, int64_t n, const double *c,
  * N = h*w, W row-major N x N (symmetric). */
 void or_grid_cost_matrix(int64_t height, int64_t width, double p, double *W);
 
+
+/* ---- feature (cell-similarity) costs -------------------------------------
+ * Points are rows of an n x d matrix stored COLUMN-major as the reference's
+ * `data::Matrix` (x[i + k*n]); metric: 0 l1, 1 l2, 2 cosine, 3 correlation
+ * (src/CellUtilities.jl:5, 78-84, 187-301). */
+#ifndef OR_FEATURE_METRICS
+#define OR_FEATURE_METRICS
+enum { OR_L1 = 0, OR_L2 = 1, OR_COSINE = 2, OR_CORRELATION = 3 };
+#endif
+/* get_flat_metrics (src/CellUtilities.jl:304-311) */
+void or_feature_stats(const double *x, int64_t n, int64_t d, double *sums,
+                      double *sqnorms, double *means, double *csqnorms);
+/* feature_cost_cuda (src/CellUtilities.jl:187-301) between row i of x and row
+ * j of y (each with its own stats) -> C, divided by `scale`. */
+double or_feature_cost(int metric, const double *x, int64_t n, int64_t i,
+                       const double *xs, const double *y, int64_t m, int64_t j,
+                       const double *ys, int64_t d, double scale);
+/* C (n x m row-major) for all pairs. */
+void or_feature_cost_matrix(int metric, const double *x, int64_t n,
+                            const double *y, int64_t m, int64_t d, double scale,
+                            double *C);
+/* extragradient_cellsim (src/solvers/MirrorProxCellSim.jl:94-188) on one
+ * data matrix (N = M = n).  nu, phi, psi: n.  res->objective is the
+ * reference's returned objective (sum of the last column costs). */
+int or_lamp_cellsim_solve(const double *data, int64_t n, int64_t d,
+                          double scale, int metric, const double *r,
+                          const double *c, const or_args *args,
+                          int64_t frequency, or_log_fn logfn, void *user,
+                          double *nu, double *phi, double *psi, or_result *res);
+/* sinkhorn_cellsim (src/solvers/SinkhornCellSim.jl:240-310): within each
+ * half-step the cost is evaluated among the rows of that call's first data
+ * argument (SinkhornCellSim.jl:43, 193); the reference calls it with
+ * data1 == data2. */
+int or_sinkhorn_cellsim_solve(const double *data1, int64_t n,
+                              const double *data2, int64_t m, int64_t d,
+                              double scale, int metric, const double *r,
+                              const double *c, const or_args *args,
+                              int64_t frequency, const double *phi0,
+                              const double *psi0, or_log_fn logfn, void *user,
+                              double *phi, double *psi, or_result *res);
+/* annealed_sinkhorn_cellsim (src/solvers/AnnealedSinkhornCellSim.jl:1-63) */
+int or_annealed_sinkhorn_cellsim_solve(const double *data1, int64_t n,
+                                       const double *data2, int64_t m,
+                                       int64_t d, double scale, int metric,
+                                       const double *r, const double *c,
+                                       const or_args *args, int64_t frequency,
+                                       or_log_fn logfn, void *user, double *phi,
+                                       double *psi, or_result *res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -261,3 +261,107 @@ def grid_cost_matrix(h, w, p):
     W = np.empty((h * w, h * w))
     lib().or_grid_cost_matrix(h, w, float(p), _p(W))
     return W
+
+
+# ---- feature (cell-similarity) costs: data are (n, d) arrays, passed to C
+# column-major as the reference's `data::Matrix` ---------------------------------
+METRICS = {"l1": 0, "l2": 1, "cosine": 2, "correlation": 3, "pearson": 3}
+SCALES = {"l1": 2.0, "l2": 2.0 ** 0.5, "cosine": 2.0, "correlation": 2.0, "pearson": 2.0}
+
+_SIG_F = {
+    "or_feature_stats": (None, [_P, _I64, _I64, _P, _P, _P, _P]),
+    "or_feature_cost_matrix": (None, [_I, _P, _I64, _P, _I64, _I64, _D, _P]),
+    "or_lamp_cellsim_solve": (_I, [_P, _I64, _I64, _D, _I, _P, _P, C.POINTER(or_args), _I64, LOG_FN,
+                                   _P, _P, _P, _P, C.POINTER(or_result)]),
+    "or_sinkhorn_cellsim_solve": (_I, [_P, _I64, _P, _I64, _I64, _D, _I, _P, _P, C.POINTER(or_args),
+                                       _I64, _P, _P, LOG_FN, _P, _P, _P, C.POINTER(or_result)]),
+    "or_annealed_sinkhorn_cellsim_solve": (_I, [_P, _I64, _P, _I64, _I64, _D, _I, _P, _P,
+                                                C.POINTER(or_args), _I64, LOG_FN, _P, _P, _P,
+                                                C.POINTER(or_result)]),
+}
+
+
+def _flib():
+    L = lib()
+    if not getattr(L, "_f_ready", False):
+        for k, (res, args) in _SIG_F.items():
+            f = getattr(L, k)
+            f.restype = res
+            f.argtypes = args
+        L._f_ready = True
+    return L
+
+
+def _colmajor(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def _fp(a):
+    assert a.dtype == np.float64 and a.flags["F_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def feature_stats(x):
+    x = _colmajor(x)
+    n, d = x.shape
+    s = [np.empty(n) for _ in range(4)]
+    _flib().or_feature_stats(_fp(x), n, d, *[_p(v) for v in s])
+    return s
+
+
+def feature_cost_matrix(x, y, metric, scale=None):
+    x, y = _colmajor(x), _colmajor(y)
+    scale = SCALES[metric] if scale is None else scale
+    C_ = np.empty((x.shape[0], y.shape[0]))
+    _flib().or_feature_cost_matrix(METRICS[metric], _fp(x), x.shape[0], _fp(y), y.shape[0],
+                                   x.shape[1], float(scale), _p(C_))
+    return C_
+
+
+def lamp_cellsim_solve(data, r, c, metric="cosine", scale=None, frequency=100, **args):
+    data = _colmajor(data)
+    r, c = _f(r), _f(c)
+    n, d = data.shape
+    scale = SCALES[metric] if scale is None else scale
+    nu, phi, psi = np.empty(n), np.empty(n), np.empty(n)
+    a, res, lg = make_args(**args), or_result(), _Log()
+    rc = _flib().or_lamp_cellsim_solve(_fp(data), n, d, float(scale), METRICS[metric], _p(r), _p(c),
+                                       C.byref(a), int(frequency), lg.cb, None, _p(nu), _p(phi),
+                                       _p(psi), C.byref(res))
+    assert rc == 0
+    return dict(nu=nu, phi=phi, psi=psi, log=lg.lines, **res.as_dict())
+
+
+def sinkhorn_cellsim_solve(data1, data2, r, c, metric="cosine", scale=None, frequency=100,
+                           phi0=None, psi0=None, **args):
+    x, y = _colmajor(data1), _colmajor(data2)
+    r, c = _f(r), _f(c)
+    n, d = x.shape
+    m = y.shape[0]
+    scale = SCALES[metric] if scale is None else scale
+    phi, psi = np.empty(n), np.empty(m)
+    p0 = None if phi0 is None else _f(phi0)
+    q0 = None if psi0 is None else _f(psi0)
+    a, res, lg = make_args(**args), or_result(), _Log()
+    rc = _flib().or_sinkhorn_cellsim_solve(_fp(x), n, _fp(y), m, d, float(scale), METRICS[metric],
+                                           _p(r), _p(c), C.byref(a), int(frequency), _p(p0), _p(q0),
+                                           lg.cb, None, _p(phi), _p(psi), C.byref(res))
+    assert rc == 0
+    return dict(phi=phi, psi=psi, log=lg.lines, **res.as_dict())
+
+
+def annealed_sinkhorn_cellsim_solve(data1, data2, r, c, metric="cosine", scale=None,
+                                    frequency=100, **args):
+    x, y = _colmajor(data1), _colmajor(data2)
+    r, c = _f(r), _f(c)
+    n, d = x.shape
+    m = y.shape[0]
+    scale = SCALES[metric] if scale is None else scale
+    phi, psi = np.empty(n), np.empty(m)
+    a, res, lg = make_args(**args), or_result(), _Log()
+    rc = _flib().or_annealed_sinkhorn_cellsim_solve(_fp(x), n, _fp(y), m, d, float(scale),
+                                                    METRICS[metric], _p(r), _p(c), C.byref(a),
+                                                    int(frequency), lg.cb, None, _p(phi), _p(psi),
+                                                    C.byref(res))
+    assert rc == 0
+    return dict(phi=phi, psi=psi, log=lg.lines, **res.as_dict())

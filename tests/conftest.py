@@ -36,6 +36,7 @@ def golden():
 def oracle():
     import oracle as o  # oracle/oracle.py (the checker)
     o.lib()
+    o.set_threads(min(os.cpu_count() or 1, 16))
     return o
 
 

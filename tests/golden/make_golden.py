@@ -127,6 +127,51 @@ def main():
             cases.append(dict(kind="theta", ref="src/solvers/MirrorProx.jl:159-180", eta_mu=eta_mu,
                               adjust=adjust, minv=minv, maxv=maxv, theta0=hx(theta0), res=hx(res),
                               c=hx(c), etamu=hx(etamu), out=out))
+    # feature costs (src/CellUtilities.jl:187-301), scales 78-84, stats 304-311
+    scales = {"l1": 2, "l2": mp.sqrt(2), "cosine": 2, "correlation": 2}
+    for metric in ("l1", "l2", "cosine", "correlation"):
+        n, m, d = 6, 5, 9
+        x = np.abs(rng.normal(size=(n, d))) * (rng.uniform(size=(n, d)) < 0.6)
+        y = np.abs(rng.normal(size=(m, d))) * (rng.uniform(size=(m, d)) < 0.6)
+        x[0, :] = 0.0
+        x[0, 2] = 1.0  # a one-hot row (zero centred norm for correlation)
+        y[1] = x[1] * 2.5  # proportional rows: cosine / correlation 1, l1 / l2 0
+        def stats(a):
+            out = []
+            for row in a:
+                v = [mp.mpf(float(t)) for t in row]
+                s = mp.fsum(v)
+                mu = s / len(v)
+                out.append((s, mp.fsum(t * t for t in v), mu, mp.fsum((t - mu) ** 2 for t in v), v))
+            return out
+        sx, sy = stats(x), stats(y)
+        sc = scales[metric]
+        C = []
+        for i in range(n):
+            row = []
+            for j in range(m):
+                si, qi, mi, ci, vi = sx[i]
+                sj, qj, mj, cj, vj = sy[j]
+                if metric == "l1":
+                    val = mp.fsum(abs(a / si - b / sj) for a, b in zip(vi, vj)) / sc
+                else:
+                    dot = mp.fsum(a * b for a, b in zip(vi, vj)) / (si * sj)
+                    if metric == "l2":
+                        val = mp.sqrt(max(qi / si ** 2 + qj / sj ** 2 - 2 * dot, 0)) / sc
+                    elif metric == "cosine":
+                        den = mp.sqrt((qi / si ** 2) * (qj / sj ** 2))
+                        val = 0 if den <= 0 else (1 - max(-1, min(1, dot / den))) / sc
+                    else:
+                        mui, muj = mi / si, mj / sj
+                        den = mp.sqrt((ci / si ** 2) * (cj / sj ** 2))
+                        if den <= 0:
+                            val = 0 if abs(mui - muj) <= mp.sqrt(mp.mpf(2) ** -52) else 1 / sc
+                        else:
+                            val = (1 - max(-1, min(1, (dot - d * mui * muj) / den))) / sc
+                row.append(float(val))
+            C.append(row)
+        cases.append(dict(kind="feature", ref="src/CellUtilities.jl:187-311", metric=metric,
+                          x=hx(x), y=hx(y), n=n, m=m, d=d, C=C))
     with open(os.path.join(HERE, "kernels.json"), "w") as f:
         json.dump({"generator": "tests/golden/make_golden.py", "dps": 50, "cases": cases}, f)
     print(f"wrote {len(cases)} cases")

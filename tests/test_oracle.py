@@ -212,3 +212,69 @@ def test_thread_count_invariance(oracle):
     b = oracle.lse_rows(x, y, th, 0.01, 1.0, 3.0, 2.0)
     oracle.set_threads(1)
     assert np.array_equal(a, b)
+
+
+def test_golden_feature_costs(golden, oracle):
+    k = 0
+    for c in golden:
+        if c["kind"] != "feature":
+            continue
+        x = unhex(c["x"], (c["n"], c["d"]))
+        y = unhex(c["y"], (c["m"], c["d"]))
+        C = oracle.feature_cost_matrix(x, y, c["metric"])
+        # l2 is sqrt(|x|^2 + |y|^2 - 2 x.y) as in the reference: an FP64
+        # cancellation of size eps becomes sqrt(eps) ~ 1.5e-8 after the root
+        tol = 5e-8 if c["metric"] == "l2" else 1e-14
+        assert np.max(np.abs(C - np.array(c["C"]))) < tol, c["metric"]
+        k += 1
+    assert k == 4
+
+
+def test_feature_cost_closed_forms(oracle):
+    """cosine: identical direction -> 0, orthogonal -> 1/2; correlation:
+    perfectly (anti)correlated -> 0 (1); l1/l2 of identical rows 0."""
+    x = np.array([[1.0, 2.0, 3.0, 0.0], [0.0, 0.0, 0.0, 5.0]])
+    y = np.array([[2.0, 4.0, 6.0, 0.0], [3.0, 2.0, 1.0, 4.0]])
+    C = oracle.feature_cost_matrix(x, y, "cosine")
+    assert abs(C[0, 0]) < 1e-15 and abs(C[1, 0] - 0.5) < 1e-15
+    a = np.array([[1.0, 2.0, 3.0, 4.0]])
+    b = np.array([[2.0, 4.0, 6.0, 8.0], [4.0, 3.0, 2.0, 1.0]])
+    Cc = oracle.feature_cost_matrix(a, b, "correlation")
+    assert abs(Cc[0, 0]) < 1e-15 and abs(Cc[0, 1] - 1.0) < 1e-14
+    # Pearson correlation of the row-sum-normalised rows equals that of the raw rows
+    from scipy.stats import pearsonr
+    rho = pearsonr(a[0], b[1])[0]
+    assert abs(Cc[0, 1] - (1 - rho) / 2) < 1e-12
+    assert np.allclose(np.diag(oracle.feature_cost_matrix(b, b, "l1")), 0.0)
+    assert np.allclose(np.diag(oracle.feature_cost_matrix(b, b, "l2")), 0.0, atol=1e-7)
+
+
+def test_cellsim_sinkhorn_matches_point_machinery(oracle):
+    """With the cosine cost precomputed, the cellsim Sinkhorn reaches the same
+    unique fixed point as the dense sinkhorn_log on C / W_inf."""
+    rng = np.random.default_rng(21)
+    X = np.abs(rng.normal(size=(30, 12))) * (rng.uniform(size=(30, 12)) < 0.5) + 1e-3
+    r, c = rng.dirichlet(np.ones(30)), rng.dirichlet(np.ones(30))
+    eta = 0.05
+    o = oracle.sinkhorn_cellsim_solve(X, X, r, c, "cosine", frequency=1, eta_p=eta, epsilon=6e-12,
+                                      itermax=100000, verbose=False)
+    C = oracle.feature_cost_matrix(X, X, "cosine")
+    W = max(C.max(), 0.0)
+    dense = oracle.sinkhorn_log_dense(r, c, C / W, frequency=1, eta_p=eta, epsilon=6e-12,
+                                      itermax=100000, verbose=False)
+    P = np.exp(-C / (eta * W) + o["phi"][:, None] + o["psi"][None, :])
+    assert np.allclose(P, dense["P"], atol=1e-10)
+
+
+def test_cellsim_lamp_stop_and_annealing(oracle):
+    rng = np.random.default_rng(22)
+    X = np.abs(rng.normal(size=(25, 10))) + 0.1
+    r, c = rng.dirichlet(np.ones(25)), rng.dirichlet(np.ones(25))
+    o = oracle.lamp_cellsim_solve(X, r, c, "correlation", frequency=10, eta_p=1e-3, epsilon=6e-6,
+                                  B=1.1, itermax=50000, verbose=False, stop="infeas")
+    assert o["converged"] == 1 and o["residual"] < 1e-6
+    a = oracle.annealed_sinkhorn_cellsim_solve(X, X, r, c, "l1", frequency=5, eta_p=0.2,
+                                               epsilon=0.05, anneal_mult=0.5, verbose=True)
+    etas = [float(l.split(",")[4]) for l in a["log"][1:]]
+    assert etas == [0.2, 0.1, 0.05]
+    assert a["log"][1].rstrip().endswith("annealed_sinkhorn_cellsim")
