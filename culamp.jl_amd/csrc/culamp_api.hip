@@ -148,7 +148,7 @@ int run_pass(culamp_ctx* ctx, const PassDesc& d, int precision, int kind) {
   LaunchCfg cfg = choose_launch(d.nt, d.ns, precision, kind, ctx->num_cu,
                                 ctx->force_rpt, ctx->force_splits);
   if (cfg.splits > 1) {
-    const size_t ntp = (size_t)cfg.tblocks * kThreads * cfg.rpt;
+    const size_t ntp = (size_t)cfg.tblocks * (kThreads / kLanesPerTarget) * cfg.rpt;
     TRY(ensure_work(ctx, (size_t)pass_fields(kind) * cfg.splits * ntp));
   }
   Work w{ctx->partial};

@@ -79,11 +79,54 @@ struct PassDesc {
   const __bf16* tz_hi; const __bf16* tz_lo;
   const __bf16* sz_hi; const __bf16* sz_lo;
   int zks;
+  // Device-resident iteration scalars for graph-captured loops: with
+  // dyn_kind != DYN_NONE the kernel reads [reg, st, mix] from dyn and derives
+  // k = -(st / (2 dyn_winf)) / reg and alpha = -1/reg (DYN_LSE) or
+  // gamma = -1/reg and mix_t = mix (DYN_COL), as the host would.
+  const double* dyn;
+  int dyn_kind;
+  double dyn_winf;
+};
+
+enum DynKind : int { DYN_NONE = 0, DYN_LSE = 1, DYN_COL = 2 };
+
+// Batched launches: one workgroup per item; instances carry their own launch
+// shape and partial-buffer base.
+struct BatchItem {
+  int32_t inst;
+  int32_t split;
+  int64_t tb;
+};
+struct BatchInst {
+  int64_t pbase;      // offset of the instance's partials in Work.partial
+  int64_t ntp;        // padded target count of the instance
+  int64_t split_len;
+  int32_t splits;
+  int32_t pad;
 };
 
 struct Work {
   double* partial;     // split partials, [field][split][target]
 };
+
+// Apply the dynamic scalars of a pass (see PassDesc::dyn).
+__device__ __forceinline__ PassDesc resolve_desc(const PassDesc& din) {
+#pragma clang fp contract(off)
+  PassDesc d = din;
+  if (d.dyn_kind != DYN_NONE) {
+    const double reg = d.dyn[0], st = d.dyn[1];
+    const double c1 = st / (2.0 * d.dyn_winf);
+    const double invreg = 1.0 / reg;
+    d.k = -c1 * invreg;
+    if (d.dyn_kind == DYN_LSE) {
+      d.alpha = -invreg;
+    } else {
+      d.gamma = -invreg;
+      d.mix_t = d.dyn[2];
+    }
+  }
+  return d;
+}
 
 struct LaunchCfg {
   int rpt;          // targets per thread
@@ -94,6 +137,7 @@ struct LaunchCfg {
 
 constexpr int kThreads = 256;
 constexpr int kTile = 256;      // sources staged in LDS per step
+constexpr int kLanesPerTarget = 4;  // point kernel: adjacent lanes sharing a target
 constexpr int kMaxFields = 5;   // partial fields per target (BARY)
 
 int pass_fields(int kind);

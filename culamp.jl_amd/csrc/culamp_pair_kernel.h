@@ -96,6 +96,56 @@ __device__ __forceinline__ R empty_max() {
   return PREC == 0 ? (R)-1e300 : (R)-1e30f;
 }
 
+// Combine two partial states of one target (commutative: the same result
+// whichever operand comes first).
+template <int PREC, int KIND, typename R>
+__device__ __forceinline__ void combine_state(TState<R>& a, const TState<R>& b) {
+  if constexpr (KIND == KIND_MAX) {
+    a.acc = a.acc > b.acc ? a.acc : b.acc;
+  } else if constexpr (KIND == KIND_MIN) {
+    a.acc = a.acc < b.acc ? a.acc : b.acc;
+  } else if constexpr (KIND == KIND_SUM) {
+    a.S += b.S;
+    a.Sc += b.Sc;
+  } else {
+    const R mn = a.m > b.m ? a.m : b.m;
+    double fa, fb;
+    if constexpr (PREC == 1) {
+      fa = ldexp(1.0, (int)fmaxf(a.m - mn, -2000.0f));
+      fb = ldexp(1.0, (int)fmaxf(b.m - mn, -2000.0f));
+    } else {
+      fa = exp((double)a.m - (double)mn);
+      fb = exp((double)b.m - (double)mn);
+    }
+    a.S = a.S * fa + b.S * fb;
+    if (KIND == KIND_BARY) {
+      a.Sy0 = a.Sy0 * fa + b.Sy0 * fb;
+      a.Sy1 = a.Sy1 * fa + b.Sy1 * fb;
+      a.Sy2 = a.Sy2 * fa + b.Sy2 * fb;
+    }
+    a.m = mn;
+  }
+}
+
+// Butterfly combine across the G adjacent lanes that share a target.
+template <int PREC, int KIND, int G, typename R>
+__device__ __forceinline__ void combine_lanes(TState<R>& st) {
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    TState<R> o = st;
+    o.m = __shfl_xor(st.m, off);
+    o.S = __shfl_xor(st.S, off);
+    o.Sc = __shfl_xor(st.Sc, off);
+    o.acc = __shfl_xor(st.acc, off);
+    if (KIND == KIND_BARY) {
+      o.Sy0 = __shfl_xor(st.Sy0, off);
+      o.Sy1 = __shfl_xor(st.Sy1, off);
+      o.Sy2 = __shfl_xor(st.Sy2, off);
+    }
+    combine_state<PREC, KIND, R>(st, o);
+  }
+}
+
 // Epilogue of one target once its state is complete.
 template <int PREC, int KIND, typename R>
 __device__ __forceinline__ void epilogue_one(const PassDesc& d, const int64_t t, const TState<R>& st) {
@@ -180,65 +230,75 @@ __device__ __forceinline__ double target_offset(const PassDesc& d, int64_t t) {
   return q;
 }
 
-// Combine the split partials of each target in split order, then apply the
-// epilogue (one thread per target).
+// Combine the split partials of each target, then apply the epilogue.  Four
+// adjacent lanes share a target and read interleaved splits (coalesced over
+// targets); they combine by butterfly, so the result is deterministic.
+constexpr int kMergeLanes = 4;
 template <int PREC, int KIND>
-__global__ __launch_bounds__(kThreads) void pass_merge(const PassDesc d, const Work w,
-                                                       const int splits, const int64_t ntp) {
+__device__ __forceinline__ void merge_body(const PassDesc& d, const Work& w, const int64_t block,
+                                           const int splits, const int64_t ntp) {
   using R = typename std::conditional<PREC == 0, double, float>::type;
-  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t >= d.nt) return;
+  constexpr int NF = kind_fields<KIND>();
+  const int sub = threadIdx.x % kMergeLanes;
+  const int64_t t = block * (kThreads / kMergeLanes) + threadIdx.x / kMergeLanes;
+  const bool ok = t < d.nt;
   TState<R> st;
   st.m = empty_max<PREC, R>();
   st.S = st.Sc = st.Sy0 = st.Sy1 = st.Sy2 = 0.0;
-  st.acc = (R)0;
-  {
-    const double q = target_offset<PREC, KIND>(d, t);
-    st.qd = PREC == 1 ? (double)(float)(q * kLog2e) : q;
-  }
-  if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
-    double a = w.partial[t];
-    for (int q = 1; q < splits; ++q) {
-      const double b = w.partial[(int64_t)q * ntp + t];
-      a = KIND == KIND_MAX ? fmax(a, b) : fmin(a, b);
-    }
-    st.acc = (R)a;
-  } else if constexpr (KIND == KIND_SUM) {
-    double a = 0.0, c = 0.0;
-    for (int q = 0; q < splits; ++q) {
-      a += w.partial[(int64_t)q * ntp + t];
-      c += w.partial[((int64_t)splits + q) * ntp + t];
-    }
-    st.S = a;
-    st.Sc = c;
-  } else {
-    double M = -INFINITY;
-    for (int q = 0; q < splits; ++q) M = fmax(M, w.partial[(int64_t)q * ntp + t]);
-    double a = 0.0, y0 = 0.0, y1 = 0.0, y2 = 0.0;
-    for (int q = 0; q < splits; ++q) {
-      const double mq = w.partial[(int64_t)q * ntp + t];
-      double fct;
-      if (PREC == 1) fct = ldexp(1.0, (int)fmax(mq - M, -2000.0));
-      else fct = exp(mq - M);
-      a += fct * w.partial[((int64_t)splits + q) * ntp + t];
-      if (KIND == KIND_BARY) {
-        y0 += fct * w.partial[((int64_t)2 * splits + q) * ntp + t];
-        y1 += fct * w.partial[((int64_t)3 * splits + q) * ntp + t];
-        y2 += fct * w.partial[((int64_t)4 * splits + q) * ntp + t];
+  st.acc = KIND == KIND_MIN ? (R)INFINITY : (R)-INFINITY;
+  if (ok) {
+    for (int q = sub; q < splits; q += kMergeLanes) {
+      TState<R> p = st;
+      if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
+        p.acc = (R)w.partial[(int64_t)q * ntp + t];
+      } else if constexpr (KIND == KIND_SUM) {
+        p.S = w.partial[(int64_t)q * ntp + t];
+        p.Sc = w.partial[((int64_t)splits + q) * ntp + t];
+      } else {
+        p.m = (R)w.partial[(int64_t)q * ntp + t];
+        p.S = w.partial[((int64_t)splits + q) * ntp + t];
+        if (NF == 5) {
+          p.Sy0 = w.partial[((int64_t)2 * splits + q) * ntp + t];
+          p.Sy1 = w.partial[((int64_t)3 * splits + q) * ntp + t];
+          p.Sy2 = w.partial[((int64_t)4 * splits + q) * ntp + t];
+        }
       }
+      combine_state<PREC, KIND, R>(st, p);
     }
-    st.m = (R)M;
-    st.S = a;
-    st.Sy0 = y0; st.Sy1 = y1; st.Sy2 = y2;
   }
+  combine_lanes<PREC, KIND, kMergeLanes, R>(st);
+  if (!ok || sub != 0) return;
+  const double q = target_offset<PREC, KIND>(d, t);
+  st.qd = PREC == 1 ? (double)(float)(q * kLog2e) : q;
   epilogue_one<PREC, KIND, R>(d, t, st);
+}
+
+template <int PREC, int KIND>
+__global__ __launch_bounds__(kThreads) void pass_merge(const PassDesc din, const Work w,
+                                                       const int splits, const int64_t ntp) {
+  const PassDesc d = resolve_desc(din);
+  merge_body<PREC, KIND>(d, w, blockIdx.x, splits, ntp);
+}
+
+// Batched merge: items[i] = (instance, merge block) for instances with splits.
+template <int PREC, int KIND>
+__global__ __launch_bounds__(kThreads) void pass_merge_batch(const PassDesc* descs,
+                                                             const BatchItem* items,
+                                                             const BatchInst* insts,
+                                                             const Work w) {
+  const BatchItem it = items[blockIdx.x];
+  const BatchInst bi = insts[it.inst];
+  const PassDesc d = resolve_desc(descs[it.inst]);
+  const Work wi{w.partial + bi.pbase};
+  merge_body<PREC, KIND>(d, wi, it.tb, bi.splits, bi.ntp);
 }
 
 template <int PREC, int KIND>
 hipError_t launch_merge(const PassDesc& d, const LaunchCfg& c, const Work& w, int64_t ntp,
                         hipStream_t s) {
   if (c.splits <= 1) return hipSuccess;
-  const unsigned blocks = (unsigned)((d.nt + kThreads - 1) / kThreads);
+  const int64_t per = kThreads / kMergeLanes;
+  const unsigned blocks = (unsigned)((d.nt + per - 1) / per);
   hipLaunchKernelGGL((pass_merge<PREC, KIND>), dim3(blocks), dim3(kThreads), 0, s, d, w,
                      c.splits, ntp);
   return hipGetLastError();
@@ -317,21 +377,19 @@ __device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], cons
 }
 
 template <int PREC, int KIND, int COST, int RPT>
-__global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
-                                                      const Work w,
-                                                      const int splits,
-                                                      const int64_t split_len) {
+__device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, const int64_t tb,
+                                          const int split, const int splits,
+                                          const int64_t split_len, const int64_t ntp) {
   using R = typename std::conditional<PREC == 0, double, float>::type;
   constexpr bool kExpKind = (KIND == KIND_LSE || KIND == KIND_SUM || KIND == KIND_BARY);
+  constexpr int G = kLanesPerTarget, TPB = kThreads / G;
 
   __shared__ R s_x[kTile], s_y[kTile], s_z[kTile], s_a[kTile], s_w[kTile];
 
   const int tid = threadIdx.x;
-  const int64_t tb = blockIdx.x;
-  const int split = blockIdx.y;
+  const int g = tid % G, tt = tid / G;
   const int64_t s0 = (int64_t)split * split_len;
   const int64_t s1 = min(d.ns, s0 + split_len);
-  const int64_t ntp = (int64_t)gridDim.x * kThreads * RPT;
   const R k = PREC == 0 ? (R)d.k : (R)(d.k * kLog2e);
   const R pexp = (R)d.p;
 
@@ -340,7 +398,7 @@ __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
   TState<R> st[RPT];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    const int64_t t = tb * kThreads * RPT + (int64_t)r * kThreads + tid;
+    const int64_t t = tb * TPB * RPT + (int64_t)r * TPB + tt;
     tidx[r] = t;
     const bool ok = t < d.nt;
     tx[r] = ok ? (R)d.tgt[3 * t] : (R)0;
@@ -362,7 +420,7 @@ __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
 
   for (int64_t base = s0; base < s1; base += kTile) {
     const int cnt = (int)min((int64_t)kTile, s1 - base);
-    const int cntp = (cnt + kSub - 1) / kSub * kSub;
+    const int cntp = (cnt + kSub * G - 1) / (kSub * G) * (kSub * G);
     __syncthreads();
     if (tid < cnt) {
       const int64_t s = base + tid;
@@ -378,7 +436,7 @@ __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
     __syncthreads();
 
     if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
-      for (int j = 0; j < cnt; ++j) {
+      for (int j = g; j < cnt; j += G) {
         const R sx = s_x[j], sy = s_y[j], sz = s_z[j], sa = s_a[j];
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
@@ -391,15 +449,18 @@ __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
         }
       }
     } else {
-      for (int j = 0; j < cntp; j += kSub) {
+      // lane g takes sources j0 + u*G + g: the G lanes of a target read G
+      // consecutive LDS words per step
+      for (int j0 = 0; j0 < cntp; j0 += kSub * G) {
         R v[RPT][kSub];
         R Cv[RPT][kSub];
         R wv[kSub], px[kSub], py[kSub], pz[kSub];
 #pragma unroll
         for (int u = 0; u < kSub; ++u) {
-          const R sx = s_x[j + u], sy = s_y[j + u], sz = s_z[j + u];
-          const R sa = s_a[j + u];
-          wv[u] = s_w[j + u];
+          const int j = j0 + u * G + g;
+          const R sx = s_x[j], sy = s_y[j], sz = s_z[j];
+          const R sa = s_a[j];
+          wv[u] = s_w[j];
           px[u] = sx; py[u] = sy; pz[u] = sz;
 #pragma unroll
           for (int r = 0; r < RPT; ++r) {
@@ -414,11 +475,44 @@ __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
       }
     }
   }
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    combine_lanes<PREC, KIND, G, R>(st[r]);
+    if (g != 0) tidx[r] = ntp;  // one owner lane per target
+  }
   finish_pass<PREC, KIND, RPT, R>(d, w, splits, split, ntp, tidx, st);
 }
 
+template <int PREC, int KIND, int COST, int RPT>
+__global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc din, const Work w,
+                                                      const int splits,
+                                                      const int64_t split_len) {
+  const PassDesc d = resolve_desc(din);
+  pair_body<PREC, KIND, COST, RPT>(d, w, blockIdx.x, blockIdx.y, splits, split_len,
+                                   (int64_t)gridDim.x * (kThreads / kLanesPerTarget) * RPT);
+}
+
+template <int PREC, int KIND, int COST, int RPT>
+__global__ __launch_bounds__(kThreads) void pair_pass_batch(const PassDesc* descs,
+                                                            const BatchItem* items,
+                                                            const BatchInst* insts,
+                                                            const Work w) {
+  const BatchItem it = items[blockIdx.x];
+  const BatchInst bi = insts[it.inst];
+  const PassDesc d = resolve_desc(descs[it.inst]);
+  const Work wi{w.partial + bi.pbase};
+  pair_body<PREC, KIND, COST, RPT>(d, wi, it.tb, it.split, bi.splits, bi.split_len, bi.ntp);
+}
 
 // ---- feature (cell-similarity) costs ------------------------------------------
+// raw_row_dot_cuda's `dot += x * y` (src/CellUtilities.jl:202-208): the product
+// is rounded before the sum (no fused multiply-add), so that the dot of a row
+// with itself equals its squared norm exactly.
+__device__ __forceinline__ double dot_step(double acc, double x, double y) {
+#pragma clang fp contract(off)
+  return acc + x * y;
+}
+
 // Cost from the accumulated statistic `acc` (sum |x~ - y~| of the rows divided
 // by their sums for l1, the raw dot x . y otherwise) and the two rows'
 // constants, with feature_cost_cuda's operations (src/CellUtilities.jl:
@@ -426,6 +520,8 @@ __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc d,
 template <int METRIC, typename R>
 __device__ __forceinline__ R feature_cost(R acc, R ni, R mi, R ci, R si, R nj, R mj, R cj, R sj,
                                           R dim, R scale) {
+  // each operation rounded on its own, as the reference evaluates it
+#pragma clang fp contract(off)
   if constexpr (METRIC == METRIC_L1) {
     return acc / scale;
   } else {
@@ -453,9 +549,10 @@ __device__ __forceinline__ R feature_cost(R acc, R ni, R mi, R ci, R si, R nj, R
 }
 
 template <int PREC, int KIND, int METRIC>
-__global__ __launch_bounds__(kThreads) void feature_pass(const PassDesc d, const Work w,
+__global__ __launch_bounds__(kThreads) void feature_pass(const PassDesc din, const Work w,
                                                          const int splits,
                                                          const int64_t split_len) {
+  const PassDesc d = resolve_desc(din);
   using R = typename std::conditional<PREC == 0, double, float>::type;
   constexpr int TT = kFeatTargets, ST = 64, KC = 16, RED = 17;
   constexpr bool kExpKind = (KIND == KIND_LSE || KIND == KIND_SUM);
@@ -554,8 +651,7 @@ __global__ __launch_bounds__(kThreads) void feature_pass(const PassDesc d, const
               const R df = xa[a] - yb[b];
               acc[a][b] += df < (R)0 ? -df : df;
             } else if constexpr (PREC == 0) {
-              // raw_row_dot_cuda's `dot += x * y`: rounded product, then sum
-              acc[a][b] = __dadd_rn(acc[a][b], __dmul_rn(xa[a], yb[b]));
+              acc[a][b] = dot_step(acc[a][b], xa[a], yb[b]);
             } else {
               acc[a][b] = fma(xa[a], yb[b], acc[a][b]);
             }
@@ -681,9 +777,9 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 template <int KIND, int NSPLIT, int KS>
-__global__ __launch_bounds__(kThreads) void mfma_feature_pass(const PassDesc d, const Work w,
-                                                              const int splits,
-                                                              const int64_t split_len) {
+__device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, const int64_t tb,
+                                          const int split, const int splits,
+                                          const int64_t split_len, const int64_t ntp) {
   constexpr int DP = 16 * KS;          // padded row width
   constexpr int LDW = DP + 8;          // LDS row stride (bf16), breaks bank alignment
   constexpr int TILE = 32;
@@ -692,11 +788,8 @@ __global__ __launch_bounds__(kThreads) void mfma_feature_pass(const PassDesc d, 
   __shared__ float s_a[TILE], s_w[TILE];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int64_t tb = blockIdx.x;
-  const int split = blockIdx.y;
   const int64_t s0 = (int64_t)split * split_len;
   const int64_t s1 = min(d.ns, s0 + split_len);
-  const int64_t ntp = (int64_t)gridDim.x * 128;
   const __bf16* Th = d.tz_hi;
   const __bf16* Tl = d.tz_lo;
   const __bf16* Sh = d.sz_hi;
@@ -816,6 +909,27 @@ __global__ __launch_bounds__(kThreads) void mfma_feature_pass(const PassDesc d, 
   finish_pass<1, KIND, 1, float>(d, w, splits, split, ntp, tidx, fin);
 }
 
+template <int KIND, int NSPLIT, int KS>
+__global__ __launch_bounds__(kThreads) void mfma_feature_pass(const PassDesc din, const Work w,
+                                                              const int splits,
+                                                              const int64_t split_len) {
+  const PassDesc d = resolve_desc(din);
+  mfma_body<KIND, NSPLIT, KS>(d, w, blockIdx.x, blockIdx.y, splits, split_len,
+                              (int64_t)gridDim.x * 128);
+}
+
+template <int KIND, int NSPLIT, int KS>
+__global__ __launch_bounds__(kThreads) void mfma_feature_pass_batch(const PassDesc* descs,
+                                                                    const BatchItem* items,
+                                                                    const BatchInst* insts,
+                                                                    const Work w) {
+  const BatchItem it = items[blockIdx.x];
+  const BatchInst bi = insts[it.inst];
+  const PassDesc d = resolve_desc(descs[it.inst]);
+  const Work wi{w.partial + bi.pbase};
+  mfma_body<KIND, NSPLIT, KS>(d, wi, it.tb, it.split, bi.splits, bi.split_len, bi.ntp);
+}
+
 template <int KIND, int NSPLIT>
 hipError_t launch_mfma_ks(const PassDesc& d, const LaunchCfg& c, const Work& w, hipStream_t s) {
   dim3 grid((unsigned)c.tblocks, (unsigned)c.splits);
@@ -842,7 +956,7 @@ hipError_t launch_rpt(const PassDesc& d, const LaunchCfg& c, const Work& w, hipS
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return launch_merge<PREC, KIND>(d, c, w, c.tblocks * kThreads * c.rpt, s);
+  return launch_merge<PREC, KIND>(d, c, w, c.tblocks * (kThreads / kLanesPerTarget) * c.rpt, s);
 }
 
 template <int PREC, int KIND>

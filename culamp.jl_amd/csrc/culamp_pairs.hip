@@ -80,19 +80,23 @@ int pass_fields(int kind) {
   return kind == KIND_BARY ? 5 : (kind == KIND_MAX || kind == KIND_MIN) ? 1 : 2;
 }
 
-// Launch shape: the largest targets-per-thread that still gives ~8
-// workgroups per CU once sources are split into chunks of >= 64 points.
+// Launch shape of the point kernel: kLanesPerTarget lanes share a target, so
+// a workgroup owns (256 / kLanesPerTarget) * rpt targets.  Take the largest
+// rpt that still gives ~8 workgroups per CU once the sources are split into
+// chunks of >= 256 points (one LDS tile); fewer, longer splits keep the
+// merge traffic small.
 LaunchCfg choose_launch(int64_t nt, int64_t ns, int precision, int kind,
                         int num_cu, int force_rpt, int force_splits) {
   LaunchCfg c{};
   const int64_t want = (int64_t)8 * (num_cu > 0 ? num_cu : 256);
+  const int64_t tpb = kThreads / kLanesPerTarget;
   const bool fp64_only = kind == KIND_BARY || kind == KIND_MAX || kind == KIND_MIN;
   const int max_rpt = (precision == 1 && !fp64_only) ? 4 : 2;
   int rpt = 1;
   int64_t splits = 1;
+  const int64_t cap = (ns + kTile - 1) / kTile;
   for (int cand = max_rpt; cand >= 1; cand /= 2) {
-    const int64_t tb = (nt + (int64_t)kThreads * cand - 1) / ((int64_t)kThreads * cand);
-    const int64_t cap = (ns + 63) / 64;
+    const int64_t tb = (nt + tpb * cand - 1) / (tpb * cand);
     int64_t sp = (want + tb - 1) / tb;
     if (sp > cap) sp = cap;
     if (sp < 1) sp = 1;
@@ -110,7 +114,7 @@ LaunchCfg choose_launch(int64_t nt, int64_t ns, int precision, int kind,
   c.rpt = rpt;
   c.splits = (int)splits;
   c.split_len = len;
-  c.tblocks = (nt + (int64_t)kThreads * rpt - 1) / ((int64_t)kThreads * rpt);
+  c.tblocks = (nt + tpb * rpt - 1) / (tpb * rpt);
   if (c.tblocks < 1) c.tblocks = 1;
   return c;
 }

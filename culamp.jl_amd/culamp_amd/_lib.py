@@ -130,7 +130,7 @@ def load() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise CulampError(CULAMP_ERR_NO_DEVICE,
                               f"{LIB_PATH} not found: build it with culamp.jl_amd/build.py")
-        lib = C.CDLL(LIB_PATH)
+        lib = C.CDLL(LIB_PATH, mode=os.RTLD_NOW | os.RTLD_LOCAL)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
             fn.restype = res
