@@ -20,44 +20,11 @@
 #include <vector>
 
 #include "culamp.h"
-#include "culamp_internal.h"
+#include "culamp_host.h"
 
 using namespace culamp;
 
-struct culamp_ctx {
-  int device = 0;
-  int num_cu = 256;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double* partial = nullptr;
-  size_t partial_cap = 0;  // doubles
-  double* scal = nullptr;      // device scalars for reductions
-  double* scal_host = nullptr; // pinned mirror
-  int force_rpt = 0, force_splits = 0;
-  double pairs = 0.0;          // pair evaluations issued (throughput accounting)
-  int64_t passes = 0;
-  std::set<culamp_problem*> problems;  // destroyed with the context
-};
-
-struct culamp_problem {
-  culamp_ctx* ctx = nullptr;
-  int64_t n = 0, m = 0;
-  double *x = nullptr, *y = nullptr, *r = nullptr, *c = nullptr;
-  double *logr = nullptr, *logc = nullptr;
-  std::vector<double> hr, hc;
-  // feature problems (cell-similarity costs); metric < 0 for point clouds
-  int metric = -1;
-  int64_t d = 0;
-  double scale = 1.0;
-  double *xf64 = nullptr, *yf64 = nullptr, *xrc = nullptr, *yrc = nullptr;
-  float *xf32 = nullptr, *yf32 = nullptr;
-  bool self = false;  // y is x
-  // MFMA path (cosine / correlation): unit rows in bf16, width 16 * zks
-  uint16_t *xz_hi = nullptr, *xz_lo = nullptr, *yz_hi = nullptr, *yz_lo = nullptr;
-  int zks = 0;  // 0: not eligible
-};
-
-namespace {
+namespace culamp_host {
 
 thread_local std::string g_err;
 
@@ -70,19 +37,6 @@ int fail(int code, const char* fmt, ...) {
   g_err = buf;
   return code;
 }
-
-#define HIP_TRY(expr)                                                          \
-  do {                                                                         \
-    hipError_t e_ = (expr);                                                    \
-    if (e_ != hipSuccess)                                                      \
-      return fail(CULAMP_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
-#define TRY(expr)                      \
-  do {                                 \
-    int rc_ = (expr);                  \
-    if (rc_ != CULAMP_OK) return rc_;  \
-  } while (0)
 
 double now_s() {
   using namespace std::chrono;
@@ -353,22 +307,6 @@ int reduce_to_host(culamp_ctx* ctx, const ReduceSpec& rs, double* out) {
   return CULAMP_OK;
 }
 
-// RAII bundle of device buffers for one solve.
-struct DevBufs {
-  std::vector<void*> ptrs;
-  ~DevBufs() {
-    for (void* p : ptrs) (void)hipFree(p);
-  }
-  int alloc(double** out, int64_t n) {
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, sizeof(double) * (size_t)(n > 0 ? n : 1));
-    if (e != hipSuccess) return fail(CULAMP_ERR_ALLOC, "hipMalloc(%lld doubles): %s", (long long)n, hipGetErrorString(e));
-    ptrs.push_back(p);
-    *out = (double*)p;
-    return CULAMP_OK;
-  }
-};
-
 int copy_out(culamp_ctx* ctx, double* host, const double* dev, int64_t n) {
   if (!host || n <= 0) return CULAMP_OK;
   HIP_TRY(hipMemcpyAsync(host, dev, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
@@ -470,7 +408,15 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
   return CULAMP_OK;
 }
 
-}  // namespace
+int need_features(const culamp_problem* pb) {
+  if (!pb) return fail(CULAMP_ERR_INVALID_ARG, "null problem");
+  if (pb->metric < 0) return fail(CULAMP_ERR_INVALID_ARG, "not a feature problem (use culamp_feature_problem_create)");
+  return CULAMP_OK;
+}
+
+}  // namespace culamp_host
+
+using namespace culamp_host;
 
 // ============================================================================
 extern "C" {
@@ -529,6 +475,7 @@ int culamp_ctx_destroy(culamp_ctx* ctx) {
   if (!ctx) return CULAMP_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  culamp_destroy_batches(ctx);
   while (!ctx->problems.empty()) culamp_problem_destroy(*ctx->problems.begin());
   if (ctx->partial) (void)hipFree(ctx->partial);
   if (ctx->scal) (void)hipFree(ctx->scal);
@@ -912,11 +859,6 @@ int culamp_feature_problem_create(culamp_ctx* ctx, const double* x, int64_t n,
   return CULAMP_OK;
 }
 
-static int need_features(const culamp_problem* pb) {
-  if (!pb) return fail(CULAMP_ERR_INVALID_ARG, "null problem");
-  if (pb->metric < 0) return fail(CULAMP_ERR_INVALID_ARG, "not a feature problem (use culamp_feature_problem_create)");
-  return CULAMP_OK;
-}
 
 // ---- feature kernel level (device vectors, problem-owned packed rows) -------
 

@@ -1,0 +1,128 @@
+// culamp_host.h -- host-side state shared by the translation units of
+// libculamp (contexts, problems, batches) and the helpers the solver loops
+// use to build and issue passes.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "culamp.h"
+#include "culamp_internal.h"
+
+struct culamp_ctx {
+  int device = 0;
+  int num_cu = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double* partial = nullptr;
+  size_t partial_cap = 0;  // doubles
+  double* scal = nullptr;      // device scalars for reductions
+  double* scal_host = nullptr; // pinned mirror
+  int force_rpt = 0, force_splits = 0;
+  double pairs = 0.0;          // pair evaluations issued (throughput accounting)
+  int64_t passes = 0;
+  std::set<culamp_problem*> problems;  // destroyed with the context
+  std::set<culamp_batch*> batches;     // destroyed with the context (first)
+};
+
+struct culamp_problem {
+  culamp_ctx* ctx = nullptr;
+  int64_t n = 0, m = 0;
+  double *x = nullptr, *y = nullptr, *r = nullptr, *c = nullptr;
+  double *logr = nullptr, *logc = nullptr;
+  std::vector<double> hr, hc;
+  // feature problems (cell-similarity costs); metric < 0 for point clouds
+  int metric = -1;
+  int64_t d = 0;
+  double scale = 1.0;
+  double *xf64 = nullptr, *yf64 = nullptr, *xrc = nullptr, *yrc = nullptr;
+  float *xf32 = nullptr, *yf32 = nullptr;
+  bool self = false;  // y is x
+  // MFMA path (cosine / correlation): unit rows in bf16, width 16 * zks
+  uint16_t *xz_hi = nullptr, *xz_lo = nullptr, *yz_hi = nullptr, *yz_lo = nullptr;
+  int zks = 0;  // 0: not eligible
+};
+
+#define HIP_TRY(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return culamp_host::fail(CULAMP_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define TRY(expr)                      \
+  do {                                 \
+    int rc_ = (expr);                  \
+    if (rc_ != CULAMP_OK) return rc_;  \
+  } while (0)
+
+namespace culamp_host {
+
+using culamp::PassDesc;
+
+extern thread_local std::string g_err;
+int fail(int code, const char* fmt, ...);
+double now_s();
+std::string fmt(const char* f, double v);
+void emit(culamp_log_fn fn, void* user, const std::string& s);
+int ensure_work(culamp_ctx* ctx, size_t partial_doubles);
+int run_pass(culamp_ctx* ctx, const PassDesc& d, int precision, int kind);
+PassDesc blank();
+void feat(PassDesc& d, const culamp_problem* pb, bool rows);
+int check_ctx(culamp_ctx* ctx);
+int need_features(const culamp_problem* pb);
+int reduce_to_host(culamp_ctx* ctx, const culamp::ReduceSpec& rs, double* out);
+int copy_out(culamp_ctx* ctx, double* host, const double* dev, int64_t n);
+double sk_aux_p(double p);
+
+PassDesc lse_rows_desc(const double* x, int64_t n, const double* y, int64_t m,
+                       const double* theta, double reg, double st, double winf,
+                       double p, double* out);
+PassDesc colmarg_desc(const double* x, int64_t n, const double* y, int64_t m,
+                      const double* r, const double* theta, const double* logZ,
+                      double reg, double st, double winf, double p, double* res,
+                      double* cost);
+PassDesc sk_half_desc(const double* x, int64_t n, const double* y, int64_t m,
+                      const double* marg, const double* psi, double reg,
+                      double winf, double p, double* out);
+PassDesc sk_resid_desc(const double* x, int64_t n, const double* y, int64_t m,
+                       const double* marg, const double* phi, const double* psi,
+                       double reg, double winf, double p, double* res,
+                       double* cost);
+PassDesc max_desc(const double* x, int64_t n, const double* y, int64_t m,
+                  double p, double* out);
+PassDesc minplus_desc(const double* x, int64_t n, const double* y, int64_t m,
+                      const double* theta, double winf, double p, double* out);
+
+// RAII bundle of device buffers for one solve.
+struct DevBufs {
+  std::vector<void*> ptrs;
+  ~DevBufs() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  int alloc(double** out, int64_t n) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (e != hipSuccess) return fail(CULAMP_ERR_ALLOC, "hipMalloc(%lld doubles): %s", (long long)n, hipGetErrorString(e));
+    ptrs.push_back(p);
+    *out = (double*)p;
+    return CULAMP_OK;
+  }
+  int alloc_bytes(void** out, size_t bytes) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes > 0 ? bytes : 1);
+    if (e != hipSuccess) return fail(CULAMP_ERR_ALLOC, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    ptrs.push_back(p);
+    *out = p;
+    return CULAMP_OK;
+  }
+};
+
+}  // namespace culamp_host
+
+// Destroys the batches of a context (defined with the batch engine).
+void culamp_destroy_batches(culamp_ctx* ctx);

@@ -109,6 +109,24 @@ struct Work {
   double* partial;     // split partials, [field][split][target]
 };
 
+// One batched pass: a PassDesc and a BatchInst per instance, and the item
+// tables of the pass and of its split merge.  Every instance of a launch
+// uses the same kernel family and template arguments.
+enum BatchFamily : int { FAMILY_POINTS = 0, FAMILY_FEATURES = 1, FAMILY_MFMA = 2 };
+struct BatchLaunch {
+  const PassDesc* descs;
+  const BatchInst* insts;
+  const BatchItem* items;
+  const BatchItem* mitems;
+  int64_t nitems, nmitems;
+  int family;
+  int rpt;     // FAMILY_POINTS: targets per thread
+  int cost;    // FAMILY_POINTS: Cost
+  int metric;  // FAMILY_FEATURES
+  int zks;     // FAMILY_MFMA: row width / 16
+  int nsplit;  // FAMILY_MFMA: 1 or 3 products
+};
+
 // Apply the dynamic scalars of a pass (see PassDesc::dyn).
 __device__ __forceinline__ PassDesc resolve_desc(const PassDesc& din) {
 #pragma clang fp contract(off)
@@ -173,6 +191,49 @@ struct ReduceSpec {
   int64_t len[8];
 };
 hipError_t launch_reduce(const ReduceSpec& spec, double* out, hipStream_t s);
+
+// Batched passes (FAMILY_POINTS / FAMILY_FEATURES in FP32 or FP64, MFMA).
+hipError_t launch_batch_fp32(const BatchLaunch& b, int kind, const Work& w, hipStream_t s);
+hipError_t launch_batch_fp64(const BatchLaunch& b, int kind, const Work& w, hipStream_t s);
+hipError_t launch_batch_mfma(const BatchLaunch& b, int kind, const Work& w, hipStream_t s);
+
+// Body of the one-block reductions: 1024 threads, strided partial sums, then
+// a fixed tree, so the result depends only on the input.
+__device__ __forceinline__ void reduce_block(const ReduceSpec& spec, double* out) {
+  __shared__ double sh[1024];
+  const int tid = threadIdx.x;
+  for (int k = 0; k < spec.nops; ++k) {
+    const int op = spec.op[k];
+    const double* a = spec.a[k];
+    const double* b = spec.b[k];
+    const int64_t n = spec.len[k];
+    double acc = op == 3 ? -INFINITY : 0.0;
+    for (int64_t i = tid; i < n; i += 1024) {
+      const double x = a[i];
+      if (op == 0) acc += x;
+      else if (op == 1) acc += fabs(x - b[i]);
+      else if (op == 2) acc += x * b[i];
+      else if (op == 3) acc = fmax(acc, x);
+      else if (op == 4) { if (x > 0.0) acc += x * log(x); }
+      else acc += fabs(x);
+    }
+    sh[tid] = acc;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+      if (tid < s) sh[tid] = op == 3 ? fmax(sh[tid], sh[tid + s]) : sh[tid] + sh[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) out[k] = sh[0];
+    __syncthreads();
+  }
+}
+
+// out = sa * a + sb * b with both products rounded (no fused multiply-add),
+// shared by every kernel that mixes potentials so they agree bit for bit.
+__device__ __forceinline__ double axpby_elem(double sa, double a, double sb, double b) {
+#pragma clang fp contract(off)
+  return sa * a + sb * b;
+}
 
 // Elementwise helpers.
 hipError_t launch_axpby(double* out, const double* a, double sa, const double* b,

@@ -13,12 +13,15 @@
 //     coalesced load per tile;
 //   * the cost is generated in registers and the max-shifted exp-sum runs on
 //     sub-tiles of 8 sources per target;
-//   * when a target's sources are split over several workgroups, each writes
-//     its partial to HBM and the last workgroup to arrive (device-scope
-//     atomic ticket, no spinning) merges the partials in split order, so
-//     results are bitwise reproducible run to run;
+//   * when a target's sources are split over several workgroups (grid.y),
+//     each writes its partial state to HBM and pass_merge, launched next on
+//     the same stream, combines the splits in a fixed order, so results are
+//     bitwise reproducible run to run;
 //   * the merged result goes through a per-target epilogue (log marginal
-//     minus LSE, marginal minus sum, fused LAMP dual update, ...).
+//     minus LSE, marginal minus sum, fused LAMP dual update, ...);
+//   * every kernel body also has a batched entry point (*_batch) whose
+//     workgroups each take one (instance, source split, target block) item
+//     from a table, so many instances of different sizes share one launch.
 //
 // Mixed precision (PREC == 1): cost and exponent in FP32, in log2 units so
 // the exponential is a single v_exp_f32.  The FP64 source term a_s is split
@@ -549,10 +552,9 @@ __device__ __forceinline__ R feature_cost(R acc, R ni, R mi, R ci, R si, R nj, R
 }
 
 template <int PREC, int KIND, int METRIC>
-__global__ __launch_bounds__(kThreads) void feature_pass(const PassDesc din, const Work w,
-                                                         const int splits,
-                                                         const int64_t split_len) {
-  const PassDesc d = resolve_desc(din);
+__device__ __forceinline__ void feature_body(const PassDesc& d, const Work& w, const int64_t tb,
+                                             const int split, const int splits,
+                                             const int64_t split_len, const int64_t ntp) {
   using R = typename std::conditional<PREC == 0, double, float>::type;
   constexpr int TT = kFeatTargets, ST = 64, KC = 16, RED = 17;
   constexpr bool kExpKind = (KIND == KIND_LSE || KIND == KIND_SUM);
@@ -564,11 +566,8 @@ __global__ __launch_bounds__(kThreads) void feature_pass(const PassDesc din, con
 
   const int tid = threadIdx.x;
   const int ty = tid >> 4, tx = tid & 15;
-  const int64_t tb = blockIdx.x;
-  const int split = blockIdx.y;
   const int64_t s0 = (int64_t)split * split_len;
   const int64_t s1 = min(d.ns, s0 + split_len);
-  const int64_t ntp = (int64_t)gridDim.x * TT;
   const R k = PREC == 0 ? (R)d.k : (R)(d.k * kLog2e);
   const R dimR = (R)d.dim, scale = (R)d.fscale;
   const R* X = PREC == 0 ? (const R*)d.tf64 : (const R*)d.tf32;
@@ -737,6 +736,27 @@ __global__ __launch_bounds__(kThreads) void feature_pass(const PassDesc din, con
     }
   }
   finish_pass<PREC, KIND, 1, R>(d, w, splits, split, ntp, tidx, fin);
+}
+
+template <int PREC, int KIND, int METRIC>
+__global__ __launch_bounds__(kThreads) void feature_pass(const PassDesc din, const Work w,
+                                                         const int splits,
+                                                         const int64_t split_len) {
+  const PassDesc d = resolve_desc(din);
+  feature_body<PREC, KIND, METRIC>(d, w, blockIdx.x, blockIdx.y, splits, split_len,
+                                   (int64_t)gridDim.x * kFeatTargets);
+}
+
+template <int PREC, int KIND, int METRIC>
+__global__ __launch_bounds__(kThreads) void feature_pass_batch(const PassDesc* descs,
+                                                               const BatchItem* items,
+                                                               const BatchInst* insts,
+                                                               const Work w) {
+  const BatchItem it = items[blockIdx.x];
+  const BatchInst bi = insts[it.inst];
+  const PassDesc d = resolve_desc(descs[it.inst]);
+  const Work wi{w.partial + bi.pbase};
+  feature_body<PREC, KIND, METRIC>(d, wi, it.tb, it.split, bi.splits, bi.split_len, bi.ntp);
 }
 
 template <int PREC, int KIND, int METRIC>
@@ -967,6 +987,76 @@ hipError_t launch_cost(const PassDesc& d, const LaunchCfg& c, const Work& w, hip
     case COST_LINF: return launch_rpt<PREC, KIND, COST_LINF>(d, c, w, s);
     default: return launch_rpt<PREC, KIND, COST_GENP>(d, c, w, s);
   }
+}
+
+// ---- batched launches: one workgroup per table item --------------------------
+template <int PREC, int KIND>
+hipError_t launch_merge_batch(const BatchLaunch& b, const Work& w, hipStream_t s) {
+  if (b.nmitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL((pass_merge_batch<PREC, KIND>), dim3((unsigned)b.nmitems), dim3(kThreads), 0,
+                     s, b.descs, b.mitems, b.insts, w);
+  return hipGetLastError();
+}
+
+template <int PREC, int KIND, int COST>
+hipError_t launch_batch_rpt(const BatchLaunch& b, const Work& w, hipStream_t s) {
+  const dim3 grid((unsigned)b.nitems);
+  switch (b.rpt) {
+    case 1: hipLaunchKernelGGL((pair_pass_batch<PREC, KIND, COST, 1>), grid, dim3(kThreads), 0, s, b.descs, b.items, b.insts, w); break;
+    case 2: hipLaunchKernelGGL((pair_pass_batch<PREC, KIND, COST, 2>), grid, dim3(kThreads), 0, s, b.descs, b.items, b.insts, w); break;
+    case 4: hipLaunchKernelGGL((pair_pass_batch<PREC, KIND, COST, 4>), grid, dim3(kThreads), 0, s, b.descs, b.items, b.insts, w); break;
+    default: return hipErrorInvalidValue;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_merge_batch<PREC, KIND>(b, w, s);
+}
+
+template <int PREC, int KIND>
+hipError_t launch_batch_points(const BatchLaunch& b, const Work& w, hipStream_t s) {
+  if (b.nitems <= 0) return hipSuccess;
+  switch (b.cost) {
+    case COST_L1: return launch_batch_rpt<PREC, KIND, COST_L1>(b, w, s);
+    case COST_SQL2: return launch_batch_rpt<PREC, KIND, COST_SQL2>(b, w, s);
+    case COST_LINF: return launch_batch_rpt<PREC, KIND, COST_LINF>(b, w, s);
+    default: return launch_batch_rpt<PREC, KIND, COST_GENP>(b, w, s);
+  }
+}
+
+template <int PREC, int KIND, int METRIC>
+hipError_t launch_batch_feature_t(const BatchLaunch& b, const Work& w, hipStream_t s) {
+  hipLaunchKernelGGL((feature_pass_batch<PREC, KIND, METRIC>), dim3((unsigned)b.nitems),
+                     dim3(kThreads), 0, s, b.descs, b.items, b.insts, w);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_merge_batch<PREC, KIND>(b, w, s);
+}
+
+template <int PREC, int KIND>
+hipError_t launch_batch_features(const BatchLaunch& b, const Work& w, hipStream_t s) {
+  if (b.nitems <= 0) return hipSuccess;
+  switch (b.metric) {
+    case METRIC_L1: return launch_batch_feature_t<PREC, KIND, METRIC_L1>(b, w, s);
+    case METRIC_L2: return launch_batch_feature_t<PREC, KIND, METRIC_L2>(b, w, s);
+    case METRIC_COSINE: return launch_batch_feature_t<PREC, KIND, METRIC_COSINE>(b, w, s);
+    default: return launch_batch_feature_t<PREC, KIND, METRIC_CORRELATION>(b, w, s);
+  }
+}
+
+template <int KIND, int NSPLIT>
+hipError_t launch_batch_mfma_ks(const BatchLaunch& b, const Work& w, hipStream_t s) {
+  if (b.nitems <= 0) return hipSuccess;
+  const dim3 grid((unsigned)b.nitems);
+  switch (b.zks) {
+    case 1: hipLaunchKernelGGL((mfma_feature_pass_batch<KIND, NSPLIT, 1>), grid, dim3(kThreads), 0, s, b.descs, b.items, b.insts, w); break;
+    case 2: hipLaunchKernelGGL((mfma_feature_pass_batch<KIND, NSPLIT, 2>), grid, dim3(kThreads), 0, s, b.descs, b.items, b.insts, w); break;
+    case 4: hipLaunchKernelGGL((mfma_feature_pass_batch<KIND, NSPLIT, 4>), grid, dim3(kThreads), 0, s, b.descs, b.items, b.insts, w); break;
+    case 8: hipLaunchKernelGGL((mfma_feature_pass_batch<KIND, NSPLIT, 8>), grid, dim3(kThreads), 0, s, b.descs, b.items, b.insts, w); break;
+    default: return hipErrorInvalidValue;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_merge_batch<1, KIND>(b, w, s);
 }
 
 }  // namespace

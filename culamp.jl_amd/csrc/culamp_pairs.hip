@@ -11,38 +11,13 @@ namespace {
 constexpr int kSub = 8;  // must match the pair kernel's sub-tile
 
 __global__ __launch_bounds__(1024) void reduce_kernel(const ReduceSpec spec, double* out) {
-  __shared__ double sh[1024];
-  const int tid = threadIdx.x;
-  for (int k = 0; k < spec.nops; ++k) {
-    const int op = spec.op[k];
-    const double* a = spec.a[k];
-    const double* b = spec.b[k];
-    const int64_t n = spec.len[k];
-    double acc = op == 3 ? -INFINITY : 0.0;
-    for (int64_t i = tid; i < n; i += 1024) {
-      const double x = a[i];
-      if (op == 0) acc += x;
-      else if (op == 1) acc += fabs(x - b[i]);
-      else if (op == 2) acc += x * b[i];
-      else if (op == 3) acc = fmax(acc, x);
-      else if (op == 4) { if (x > 0.0) acc += x * log(x); }
-      else acc += fabs(x);
-    }
-    sh[tid] = acc;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-      if (tid < s) sh[tid] = op == 3 ? fmax(sh[tid], sh[tid + s]) : sh[tid] + sh[tid + s];
-      __syncthreads();
-    }
-    if (tid == 0) out[k] = sh[0];
-    __syncthreads();
-  }
+  reduce_block(spec, out);
 }
 
 __global__ void axpby_kernel(double* out, const double* a, double sa,
                              const double* b, double sb, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = sa * a[i] + sb * b[i];
+  if (i < n) out[i] = axpby_elem(sa, a[i], sb, b[i]);
 }
 __global__ void log_kernel(double* out, const double* a, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -15,6 +15,7 @@ from .solvers import (annealed_sinkhorn_euclidean, default_context,
                       extragradient_color_transfer, extragradient_euclidean,
                       sinkhorn_color_transfer, sinkhorn_euclidean, Problem)
 from . import kernels
+from .batch import Batch
 from .cellsim import (FeatureProblem, annealed_sinkhorn_cellsim, cost_scale,
                       extragradient_cellsim, normalize_cell_cost_metric, sinkhorn_cellsim)
 
@@ -24,5 +25,5 @@ __all__ = [
     "default_context", "extragradient_color_transfer", "extragradient_euclidean",
     "sinkhorn_color_transfer", "sinkhorn_euclidean", "Problem", "kernels", "FeatureProblem",
     "annealed_sinkhorn_cellsim", "cost_scale", "extragradient_cellsim",
-    "normalize_cell_cost_metric", "sinkhorn_cellsim",
+    "normalize_cell_cost_metric", "sinkhorn_cellsim", "Batch",
 ]

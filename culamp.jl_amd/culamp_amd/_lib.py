@@ -60,6 +60,19 @@ class culamp_result(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class culamp_instance(C.Structure):
+    _fields_ = [("n", C.c_int64), ("m", C.c_int64), ("x", C.c_void_p), ("y", C.c_void_p),
+                ("r", C.c_void_p), ("c", C.c_void_p)]
+
+
+class culamp_batch_tuning(C.Structure):
+    _fields_ = [("rows_per_thread", C.c_int32), ("graph", C.c_int32),
+                ("max_split", C.c_int64), ("min_items", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 LOG_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)
 
 _P = C.c_void_p
@@ -115,6 +128,17 @@ SIGNATURES = {
                                            _P, _P, _P, C.POINTER(culamp_result)]),
     "culamp_annealed_sinkhorn_cellsim_solve": (_I, [_P, _P, C.POINTER(culamp_args), _I64, LOG_FN,
                                                     _P, _P, _P, C.POINTER(culamp_result)]),
+    "culamp_batch_create": (_I, [_P, C.POINTER(culamp_instance), C.c_int32, _I64, _I, _D,
+                                 C.POINTER(_P)]),
+    "culamp_batch_destroy": (_I, [_P]),
+    "culamp_batch_set_tuning": (_I, [_P, C.POINTER(culamp_batch_tuning)]),
+    "culamp_batch_get_tuning": (_I, [_P, C.POINTER(culamp_batch_tuning)]),
+    "culamp_batch_autotune": (_I, [_P, _P, _I, _D, _I, C.POINTER(culamp_batch_tuning),
+                                   C.POINTER(_D)]),
+    "culamp_batch_sinkhorn_solve": (_I, [_P, _P, C.POINTER(culamp_args), _I64, _D, _P, _P, _P, _P,
+                                         C.POINTER(culamp_result)]),
+    "culamp_batch_lamp_solve": (_I, [_P, _P, C.POINTER(culamp_args), _I64, _D, _P, _P, _P,
+                                     C.POINTER(culamp_result)]),
 }
 
 _lib = None

@@ -67,6 +67,7 @@ enum {
 
 typedef struct culamp_ctx culamp_ctx;
 typedef struct culamp_problem culamp_problem;
+typedef struct culamp_batch culamp_batch;
 
 /* Mirrors EOTArgs (src/Utilities.jl:18-31) field for field, same defaults via
  * culamp_args_default.  Two fields are new and default to the reference's
@@ -288,6 +289,69 @@ int culamp_annealed_sinkhorn_cellsim_solve(culamp_ctx *ctx,
                                            culamp_log_fn log_fn, void *user,
                                            double *phi, double *psi,
                                            culamp_result *result);
+
+/* ---- batches: many independent instances per launch ----------------------
+ * No single reference symbol: the reference solves one instance per call and
+ * its only multi-instance workload loops over pairs on the host
+ * (src/CellSimilarity.jl:164-255).  A batch runs every instance's passes in
+ * one launch per pass (one workgroup per (instance, source split, target
+ * block) item, largest items first), checks all instances every `frequency`
+ * iterations and drops converged ones; the iterations between checks are
+ * captured once per active set in a HIP graph and replayed.  Per instance the
+ * arithmetic, stop test and outputs are those of the single-instance solver
+ * named below (FP64: equal to it up to the summation order of split passes). */
+
+/* One instance (host arrays, copied at culamp_batch_create). */
+typedef struct {
+  int64_t n, m;
+  const double *x; /* point clouds: 3 x n; features: n x d column-major */
+  const double *y; /* 3 x m, or m x d; features may pass NULL for y = x */
+  const double *r; /* n */
+  const double *c; /* m */
+} culamp_instance;
+
+/* Launch shape of a batch.  0 selects automatically. */
+typedef struct {
+  int32_t rows_per_thread; /* point kernel targets per thread: 1, 2 or 4 */
+  int32_t graph;           /* 1: replay captured iterations (default), 0: eager */
+  int64_t max_split;       /* most sources one workgroup reads per pass */
+  int64_t min_items;       /* fewest workgroups a pass should have */
+} culamp_batch_tuning;
+
+/* d == 0: point clouds (3-D, cost chosen by p at solve time).  d > 0:
+ * feature instances with `metric` and `scale` as culamp_feature_problem_create.
+ * count <= 65535. */
+int culamp_batch_create(culamp_ctx *ctx, const culamp_instance *inst,
+                        int32_t count, int64_t d, int metric, double scale,
+                        culamp_batch **out);
+int culamp_batch_destroy(culamp_batch *batch);
+int culamp_batch_set_tuning(culamp_batch *batch, const culamp_batch_tuning *t);
+int culamp_batch_get_tuning(const culamp_batch *batch, culamp_batch_tuning *t);
+/* Times candidate shapes of the batch's row log-sum-exp pass at `precision`
+ * (HIP events, `reps` launches each) and keeps the fastest; *chosen (may be
+ * NULL) receives it and *ms_per_pass (may be NULL) its time. */
+int culamp_batch_autotune(culamp_ctx *ctx, culamp_batch *batch, int precision,
+                          double p, int reps, culamp_batch_tuning *chosen,
+                          double *ms_per_pass);
+/* Per instance: sinkhorn_color_transfer (points, SinkhornKernel.jl:477-559)
+ * or sinkhorn_cellsim (features, SinkhornCellSim.jl:240-310).  phi0/psi0
+ * (optional) and phi/psi are the instances' vectors concatenated in order
+ * (sum n and sum m doubles); results has `count` entries.  `verbose` keeps
+ * its meaning for the stop test; no log lines are written.  tmax is tested
+ * between checks. */
+int culamp_batch_sinkhorn_solve(culamp_ctx *ctx, culamp_batch *batch,
+                                const culamp_args *args, int64_t frequency,
+                                double p, const double *phi0,
+                                const double *psi0, double *phi, double *psi,
+                                culamp_result *results);
+/* Per instance: extragradient_color_transfer without images (points,
+ * MirrorProxKernel.jl:1019-1132) or extragradient_cellsim (features,
+ * MirrorProxCellSim.jl:94-188).  nu, psi: sum m; phi: sum n (any may be
+ * NULL). */
+int culamp_batch_lamp_solve(culamp_ctx *ctx, culamp_batch *batch,
+                            const culamp_args *args, int64_t frequency,
+                            double p, double *nu, double *phi, double *psi,
+                            culamp_result *results);
 
 #ifdef __cplusplus
 }
