@@ -40,6 +40,79 @@ def pass_bytes(n: int, m: int, b_c: int, b_x: int = 8) -> float:
     return n * m * b_c + (n + m) * 3 * b_x + 8 * (n + m) + 8 * n
 
 
+def bench_batched(ctx, a, log) -> dict:
+    """Cell-similarity batch: `count` independent instances (sizes log-uniform
+    in [256, 4096], d = 128 sparse unit rows, cosine cost / 2, eta_p = 5e-2)
+    solved to L1 infeasibility 1e-6 in one batch (bf16 MFMA cost tiles,
+    FP64 potentials), against the same instances solved one after the other
+    by the plain FP64 single-instance path (host-checked stop every
+    `frequency` iterations, no batching, no graph)."""
+    import numpy as np
+    import culamp_amd
+    from culamp_amd import workloads
+    insts = workloads.cell_batch(a.batch_count)
+    b = culamp_amd.Batch(insts, ctx, metric="cosine", scale=2.0)
+    prec = a.batch_precision
+    tune, tune_ms = b.autotune(prec, reps=3)
+    sk = culamp_amd.EOTArgs(eta_p=5e-2, epsilon=6e-6, itermax=20000, verbose=False, precision=prec)
+    lp = sk.copy(stop="infeas", B=1.1)
+    out = {"workload": f"{len(insts)} cell-similarity instances, n,m log-uniform in [256, 4096], "
+                       "d=128 |N(0,1)|*Bernoulli(0.3) unit rows, cosine/2, Dirichlet(1) weights, "
+                       "eta_p=5e-2, stop at L1 infeasibility 1e-6",
+           "precision": prec, "tuning": tune, "pairs_per_pass": float(sum(
+               x.shape[0] * y.shape[0] for x, y, _, _ in insts))}
+    for name, fn, args in (("sinkhorn", b.sinkhorn, sk), ("lamp", b.lamp, lp)):
+        fn(args, a.frequency)  # warm-up (graph capture, allocations)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        res = fn(args, a.frequency)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        infos = [r[-1] for r in res]
+        pairs = sum(i["pairs"] for i in infos)
+        out[name] = {
+            "solves_per_s": len(insts) / dt, "seconds": dt,
+            "converged": int(sum(i["converged"] for i in infos)),
+            "max_infeasibility": max(i["residual"] for i in infos),
+            "iterations_mean": float(np.mean([i["iterations"] for i in infos])),
+            "gbs_dense_eq": pairs * 4 / 1e9 / dt,
+            "mfma_tflops": pairs * 2 * 128 * (3 if prec == "bf16x3" else 1) / 1e12 / dt,
+            "objectives": [i["objective"] for i in infos],
+        }
+        log(f"batched {name}: {len(insts)} solves in {dt:.3f}s, "
+            f"{out[name]['converged']} converged, max infeas {out[name]['max_infeasibility']:.2e}")
+    # plain FP64 single-instance solves, one after the other (inputs resident)
+    fp = sk.copy(precision="fp64")
+    fpl = lp.copy(precision="fp64")
+    pbs = [culamp_amd.FeatureProblem(x, y, 2.0, "cosine", r, c, ctx) for x, y, r, c in insts]
+    for name, args in (("sinkhorn", fp), ("lamp", fpl)):
+        objs, done = [], 0
+        t0 = time.perf_counter()
+        for pb in pbs:
+            if name == "sinkhorn":
+                _, _, info = pb.sinkhorn(args, a.frequency, log=None)
+            else:
+                _, _, _, info = pb.lamp(args, a.frequency, log=None)
+            objs.append(info["objective"])
+            done += 1
+            if time.perf_counter() - t0 > a.plain_seconds:
+                break
+        dt = time.perf_counter() - t0
+        rate = done / dt
+        bo = out[name]["objectives"][:done]
+        rel = max(abs(u - v) / abs(v) for u, v in zip(bo, objs))
+        out[name].update({"plain_fp64_solves_per_s": rate, "plain_fp64_instances_timed": done,
+                          "speedup_vs_plain_fp64": out[name]["solves_per_s"] / rate,
+                          "objective_max_rel_err_vs_fp64": rel})
+        del out[name]["objectives"]
+        log(f"plain fp64 {name}: {done} solves in {dt:.2f}s; batched speedup "
+            f"{out[name]['speedup_vs_plain_fp64']:.1f}x, objective rel err {rel:.2e}")
+    for pb in pbs:
+        pb.close()
+    b.close()
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +125,11 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--kernel-launches", type=int, default=200)
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-precision", default="bf16")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--plain-seconds", type=float, default=60.0,
+                    help="time budget of the sequential FP64 single-instance comparison")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
@@ -96,7 +174,7 @@ def main() -> None:
     out_fp64 = {}
     if not a.no_fp64:
         t1 = time.perf_counter()
-        _, _, _, _, _, i64 = pb.lamp(fp64, 100, 2.0, log=None, assignments=False)
+        _, _, _, _, _, i64 = pb.lamp(fp64, a.frequency, 2.0, log=None, assignments=False)
         d64 = time.perf_counter() - t1
         out_fp64 = {
             "fp64_plain_solves_per_s": 1.0 / d64,
@@ -153,6 +231,8 @@ def main() -> None:
                "iterations_per_s": it_c / dc, "pairs_per_s": cpu_pairs / dc}
         log(f"cpu oracle: {it_c} it in {dc:.2f}s")
 
+    batched = None if a.no_batch else bench_batched(ctx, a, log)
+
     ms_per_step = dt / a.steps * 1e3
     rec = {
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
@@ -168,6 +248,8 @@ def main() -> None:
         "roofline": roofline, "cpu_baseline": cpu,
     }
     rec.update(out_fp64)
+    if batched is not None:
+        rec["batched"] = batched
     print(json.dumps(rec), flush=True)
 
 

@@ -1026,6 +1026,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
       d.th0 = theta; d.cvec = pb->c; d.etamu = etamu; d.eta_mu = a->eta_mu;
       d.minv = minv; d.maxv = maxv; d.upd_winf = 1.0; d.th_out = thetab;
       d.nu_in = nu; d.mix_src = theta; d.nu_out = nub; d.mix_t = etat_new;
+      d.out1 = nullptr;  // the column costs are only read at checks
       TRY(run_pass(ctx, d, prec, KIND_SUM));
     }
     etat = etat_new;
@@ -1037,6 +1038,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
       d.th0 = theta; d.cvec = pb->c; d.etamu = etamu; d.eta_mu = a->eta_mu;
       d.minv = minv; d.maxv = maxv; d.upd_winf = 1.0; d.th_out = theta;
       d.nu_in = nu; d.mix_src = thetab; d.nu_out = nu; d.mix_t = etat;
+      d.out1 = nullptr;
       TRY(run_pass(ctx, d, prec, KIND_SUM));
     }
     steps += 1;
@@ -1360,8 +1362,10 @@ int culamp_lamp_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_
     HIP_TRY(hipStreamSynchronize(ctx->stream));
   }
   if (phi_out) {
-    HIP_TRY(launch_log_minus(cost, pb->r, sumvals, n, ctx->stream));
-    TRY(copy_out(ctx, phi_out, cost, n));
+    double* phi_dev;
+    TRY(B.alloc(&phi_dev, n));
+    HIP_TRY(launch_log_minus(phi_dev, pb->r, sumvals, n, ctx->stream));
+    TRY(copy_out(ctx, phi_out, phi_dev, n));
   }
   TRY(copy_out(ctx, nu_out, nu, m));
   HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -39,7 +39,8 @@ enum Epi : int {
   EPI_LAMP_COL = 4,  // out0[t] = S, out1[t] = Sc, then the fused theta / nu update
   EPI_BARY = 5,      // out0[3t + k] = weighted mean of source coordinate k
   EPI_MAX = 6,       // out0[t] = max
-  EPI_MIN = 7        // out0[t] = min
+  EPI_MIN = 7,       // out0[t] = min
+  EPI_ADJ = 8        // KIND_BARY state, adjoint of an LSE pass (see PassDesc::adj_*)
 };
 
 struct PassDesc {
@@ -86,6 +87,15 @@ struct PassDesc {
   const double* dyn;
   int dyn_kind;
   double dyn_winf;
+  // EPI_ADJ (squared-Euclidean gradients of an LSE pass), with the pass's
+  // S_t = sum_s w_s e^{v_ts} and Y_t = sum_s w_s e^{v_ts} pos_s:
+  //   adj_norm == 0: out1[t] = adj_beta out1[t] + adj_c1 S_t
+  //                  out0[3t+k] += adj_c0 (tgt_k S_t - Y_tk)
+  //   adj_norm == 1: out0[3t+k] += adj_c0 tgtW[t] (tgt_k - Y_tk / S_t)
+  // (out0 / out1 may be null).
+  const double* tgtW;
+  double adj_c0, adj_c1, adj_beta;
+  int adj_norm;
 };
 
 enum DynKind : int { DYN_NONE = 0, DYN_LSE = 1, DYN_COL = 2 };

@@ -155,9 +155,30 @@ __device__ __forceinline__ void epilogue_one(const PassDesc& d, const int64_t t,
   if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
     d.out0[t] = (double)st.acc;
   } else if constexpr (KIND == KIND_BARY) {
-    d.out0[3 * t] = st.Sy0 / st.S;
-    d.out0[3 * t + 1] = st.Sy1 / st.S;
-    d.out0[3 * t + 2] = st.Sy2 / st.S;
+    if (d.epi == EPI_ADJ) {
+      const double p0 = d.tgt[3 * t], p1 = d.tgt[3 * t + 1], p2 = d.tgt[3 * t + 2];
+      if (d.adj_norm) {
+        const double w = (d.tgtW != nullptr ? d.tgtW[t] : 1.0) * d.adj_c0;
+        const double inv = 1.0 / st.S;
+        d.out0[3 * t] += w * (p0 - st.Sy0 * inv);
+        d.out0[3 * t + 1] += w * (p1 - st.Sy1 * inv);
+        d.out0[3 * t + 2] += w * (p2 - st.Sy2 * inv);
+      } else {
+        const double f = exp((double)st.m + (d.tgtA != nullptr ? d.gamma * d.tgtA[t] : 0.0));
+        const double S = st.S * f;
+        if (d.out1 != nullptr)
+          d.out1[t] = d.adj_beta != 0.0 ? d.adj_beta * d.out1[t] + d.adj_c1 * S : d.adj_c1 * S;
+        if (d.out0 != nullptr) {
+          d.out0[3 * t] += d.adj_c0 * (p0 * S - st.Sy0 * f);
+          d.out0[3 * t + 1] += d.adj_c0 * (p1 * S - st.Sy1 * f);
+          d.out0[3 * t + 2] += d.adj_c0 * (p2 * S - st.Sy2 * f);
+        }
+      }
+    } else {
+      d.out0[3 * t] = st.Sy0 / st.S;
+      d.out0[3 * t + 1] = st.Sy1 / st.S;
+      d.out0[3 * t + 2] = st.Sy2 / st.S;
+    }
   } else if constexpr (KIND == KIND_LSE) {
     double lse;
     if ((double)st.m <= (double)empty_max<PREC, R>() || st.S == 0.0) lse = -INFINITY;
@@ -1018,6 +1039,7 @@ hipError_t launch_batch_points(const BatchLaunch& b, const Work& w, hipStream_t 
   switch (b.cost) {
     case COST_L1: return launch_batch_rpt<PREC, KIND, COST_L1>(b, w, s);
     case COST_SQL2: return launch_batch_rpt<PREC, KIND, COST_SQL2>(b, w, s);
+
     case COST_LINF: return launch_batch_rpt<PREC, KIND, COST_LINF>(b, w, s);
     default: return launch_batch_rpt<PREC, KIND, COST_GENP>(b, w, s);
   }

@@ -16,6 +16,8 @@ from .solvers import (annealed_sinkhorn_euclidean, default_context,
                       sinkhorn_color_transfer, sinkhorn_euclidean, Problem)
 from . import kernels
 from .batch import Batch
+from .diff import divergence_opts, sinkhorn_divergence
+from . import diff
 from .cellsim import (FeatureProblem, annealed_sinkhorn_cellsim, cost_scale,
                       extragradient_cellsim, normalize_cell_cost_metric, sinkhorn_cellsim)
 
@@ -25,5 +27,6 @@ __all__ = [
     "default_context", "extragradient_color_transfer", "extragradient_euclidean",
     "sinkhorn_color_transfer", "sinkhorn_euclidean", "Problem", "kernels", "FeatureProblem",
     "annealed_sinkhorn_cellsim", "cost_scale", "extragradient_cellsim",
-    "normalize_cell_cost_metric", "sinkhorn_cellsim", "Batch",
+    "normalize_cell_cost_metric", "sinkhorn_cellsim", "Batch", "divergence_opts",
+    "sinkhorn_divergence", "diff",
 ]

@@ -73,6 +73,12 @@ class culamp_batch_tuning(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class culamp_divergence_opts(C.Structure):
+    _fields_ = [("eps", C.c_double), ("eta_p", C.c_double), ("tol", C.c_double),
+                ("itermax", C.c_int64), ("frequency", C.c_int64), ("unroll", C.c_int32),
+                ("checkpoint", C.c_int32), ("precision", C.c_int32), ("reserved", C.c_int32)]
+
+
 LOG_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)
 
 _P = C.c_void_p
@@ -139,6 +145,14 @@ SIGNATURES = {
                                          C.POINTER(culamp_result)]),
     "culamp_batch_lamp_solve": (_I, [_P, _P, C.POINTER(culamp_args), _I64, _D, _P, _P, _P,
                                      C.POINTER(culamp_result)]),
+    "culamp_divergence_opts_default": (None, [C.POINTER(culamp_divergence_opts)]),
+    "culamp_sinkhorn_divergence": (_I, [_P, _P, _I64, _P, _P, _I64, _P,
+                                        C.POINTER(culamp_divergence_opts), C.POINTER(_D), _P, _P,
+                                        _P, _P, _P]),
+    "culamp_sinkhorn_divergence_vjp": (_I, [_P, _P, _I64, _P, _P, _I64, _P,
+                                            C.POINTER(culamp_divergence_opts), _D, _P, _P, _P, _P]),
+    "culamp_lse_adjoint": (_I, [_P, _P, _I64, _P, _I64, _D, _P, _D, _P, _P, _D, _I, _P, _D, _D,
+                                _D, _P, _P]),
 }
 
 _lib = None

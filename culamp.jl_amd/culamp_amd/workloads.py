@@ -9,6 +9,11 @@
 * ``uniform_grid_pair`` -- two g x g histograms, entries Uniform(0,1)+1e-3.
 * ``gmm_clouds``       -- two RGB-like point clouds in [0,1]^3 drawn from
   Gaussian mixtures, uniform weights.
+* ``cell_batch``       -- independent cell-similarity instances: sizes
+  log-uniform in [lo, hi], d-dimensional rows |N(0,1)| * Bernoulli(density),
+  L2-normalised (all-zero rows redrawn), Dirichlet(1) weights.
+* ``divergence_clouds`` -- N(0, I_3) source, 4-component Gaussian-mixture
+  target, Uniform(0.5, 1.5) weights normalised.
 """
 from __future__ import annotations
 
@@ -67,3 +72,37 @@ def gmm_clouds(n: int = 16384, m: int = 16384, seed_x: int = 301, seed_y: int = 
     x = gmm_cloud(np.random.default_rng(seed_x), n, 5)
     y = gmm_cloud(np.random.default_rng(seed_y), m, 8)
     return x, y, np.full(n, 1.0 / n), np.full(m, 1.0 / m)
+
+
+def sparse_unit_rows(rng: np.random.Generator, n: int, d: int, density: float) -> np.ndarray:
+    x = np.abs(rng.normal(size=(n, d))) * (rng.uniform(size=(n, d)) < density)
+    bad = ~(x.sum(axis=1) > 0)
+    while bad.any():
+        x[bad] = np.abs(rng.normal(size=(bad.sum(), d))) * (rng.uniform(size=(bad.sum(), d)) < density)
+        bad = ~(x.sum(axis=1) > 0)
+    return x / np.linalg.norm(x, axis=1, keepdims=True)
+
+
+def cell_batch(count: int = 256, lo: int = 256, hi: int = 4096, d: int = 128, density: float = 0.3,
+               seed: int = 401):
+    """List of (x (n, d), y (m, d), r, c) instances."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n, m = (int(np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi))))) for _ in range(2))
+        x = sparse_unit_rows(rng, n, d, density)
+        y = sparse_unit_rows(rng, m, d, density)
+        out.append((x, y, rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(m))))
+    return out
+
+
+def divergence_clouds(n: int = 65536, m: int = 65536, seed_x: int = 501, seed_y: int = 502):
+    rx, ry = np.random.default_rng(seed_x), np.random.default_rng(seed_y)
+    x = rx.normal(size=(n, 3))
+    centres = ry.normal(0, 2.0, size=(4, 3))
+    sig = ry.uniform(0.2, 0.6, size=4)
+    comp = ry.choice(4, size=m, p=ry.dirichlet(np.ones(4)))
+    y = centres[comp] + ry.normal(size=(m, 3)) * sig[comp, None]
+    a = rx.uniform(0.5, 1.5, n)
+    b = ry.uniform(0.5, 1.5, m)
+    return x, y, a / a.sum(), b / b.sum()

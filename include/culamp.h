@@ -353,6 +353,65 @@ int culamp_batch_lamp_solve(culamp_ctx *ctx, culamp_batch *batch,
                             double p, double *nu, double *phi, double *psi,
                             culamp_result *results);
 
+/* ---- differentiation: Sinkhorn divergence and its gradient ---------------
+ * No reference counterpart (the reference has no gradients).  For squared
+ * Euclidean costs C_ij = |x_i - y_j|^2 and a fixed regularisation eps,
+ *   OT(a,x; b,y) = eps [<a,phi> + <b,psi> - <a,log a> - <b,log b>
+ *                       - sum_ij P_ij + 1],   P_ij = e^{-C_ij/eps + phi_i + psi_j}
+ * at the Sinkhorn potentials (the reference's log-domain convention,
+ * SinkhornKernel.jl:341-444, with reg * W_inf = eps), and
+ *   S = OT(a,x; b,y) - OT(a,x; a,x)/2 - OT(b,y; b,y)/2.
+ * Gradients: the envelope term (partials at fixed potentials) plus the
+ * adjoint of the last `unroll` Sinkhorn iterations of each solve
+ * (truncated back-propagation through the log-sum-exp passes), recomputing
+ * the forward from checkpoints stored every `checkpoint` iterations.  The
+ * gradient in a is the unprojected partial derivative; only its projection
+ * on {sum a = 0} is meaningful for normalised weights. */
+typedef struct {
+  double eps;         /* > 0: regularisation; <= 0: eta_p * max_ij C(x_i, y_j) */
+  double eta_p;       /* used when eps <= 0 */
+  double tol;         /* stop: L1 row infeasibility <= tol */
+  int64_t itermax;
+  int64_t frequency;  /* infeasibility tested every `frequency` iterations */
+  int32_t unroll;     /* iterations back-propagated at the end of each solve (0: envelope only) */
+  int32_t checkpoint; /* checkpoint interval inside the unrolled window (>= 1) */
+  int32_t precision;  /* forward passes: CULAMP_FP64 or CULAMP_MIXED; adjoint passes are FP64 */
+  int32_t reserved;
+} culamp_divergence_opts;
+
+void culamp_divergence_opts_default(culamp_divergence_opts *opts);
+/* value (may be NULL); gradients (any may be NULL): grad_x 3 x n, grad_a n,
+ * grad_y 3 x m, grad_b m; terms (3 entries or NULL): the OT(a,b), OT(a,a)
+ * and OT(b,b) solves (objective = the term's value). */
+int culamp_sinkhorn_divergence(culamp_ctx *ctx, const double *x, int64_t n,
+                               const double *a, const double *y, int64_t m,
+                               const double *b,
+                               const culamp_divergence_opts *opts,
+                               double *value, double *grad_x, double *grad_a,
+                               double *grad_y, double *grad_b,
+                               culamp_result *terms);
+/* Vector-Jacobian product: the gradients above times the cotangent dS. */
+int culamp_sinkhorn_divergence_vjp(culamp_ctx *ctx, const double *x, int64_t n,
+                                   const double *a, const double *y, int64_t m,
+                                   const double *b,
+                                   const culamp_divergence_opts *opts, double dS,
+                                   double *grad_x, double *grad_a,
+                                   double *grad_y, double *grad_b);
+/* Kernel level (device pointers): one adjoint pass over targets t (3 x nt)
+ * and sources s (3 x ns) with v_ts = -|t - s|^2 / eps + alpha srcA[s]
+ * + gamma tgtA[t] and weights srcB[s] (NULL: 1):
+ *   S_t = sum_s srcB[s] e^{v_ts},  Y_t = sum_s srcB[s] e^{v_ts} s
+ *   normalized == 0: out1[t] = beta out1[t] + c1 S_t (beta 0: overwrite),
+ *                    out0[3t+k] += c0 (t_k S_t - Y_tk)
+ *   normalized != 0: out0[3t+k] += c0 tgtW[t] (t_k - Y_tk / S_t)
+ * srcA, tgtA, tgtW, out0, out1 may be NULL. */
+int culamp_lse_adjoint(culamp_ctx *ctx, const double *tgt, int64_t nt,
+                       const double *src, int64_t ns, double eps,
+                       const double *srcA, double alpha, const double *srcB,
+                       const double *tgtA, double gamma, int normalized,
+                       const double *tgtW, double c0, double c1, double beta,
+                       double *out0, double *out1);
+
 #ifdef __cplusplus
 }
 #endif

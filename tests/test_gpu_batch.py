@@ -204,3 +204,23 @@ def test_batch_mixed_points(gpu_ctx, oracle):
             assert info["converged"] == 1 and info["residual"] <= 1e-6
             assert abs(info["objective"] - o["objective"]) / o["objective"] < 1e-5
     b.close()
+
+
+def test_cellsim_lamp_ragged_single_vs_batch(gpu_ctx):
+    """extragradient_cellsim generalised to x != y (n != m): the single-instance
+    solver and the batch agree (FP64), including phi (length n)."""
+    import culamp_amd
+    rng = np.random.default_rng(30)
+    x, y = cells(rng, 70, 12), cells(rng, 45, 12)
+    r, c = rng.dirichlet(np.ones(70)), rng.dirichlet(np.ones(45))
+    kw = dict(eta_p=1e-2, epsilon=6e-6, B=1.1, itermax=20000, verbose=False, stop="infeas")
+    pb = culamp_amd.FeatureProblem(x, y, 2.0, "cosine", r, c, gpu_ctx)
+    nu, phi, psi, info = pb.lamp(culamp_amd.EOTArgs(**kw), 10, log=None)
+    b = culamp_amd.Batch([(x, y, r, c)], gpu_ctx, metric="cosine", scale=2.0)
+    (bnu, bphi, bpsi, binfo), = b.lamp(culamp_amd.EOTArgs(**kw), 10)
+    assert info["converged"] == 1 and info["iterations"] == binfo["iterations"]
+    assert phi.shape == (70,) and np.all(np.isfinite(phi))
+    assert np.allclose(nu, bnu, atol=1e-12) and np.allclose(phi, bphi, atol=1e-10)
+    assert np.allclose(psi, bpsi, atol=1e-10)
+    b.close()
+    pb.close()
