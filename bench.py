@@ -113,6 +113,62 @@ def bench_batched(ctx, a, log) -> dict:
     return out
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_lse_mixed_grid64.json")
+
+
+def lse_roofline(ctx, x, y, theta, a, log) -> dict:
+    """Roofline of the dominant kernel: the mixed row LSE (pair_pass + split
+    merge, culamp_lse_rows) timed with HIP events on the library's stream."""
+    import culamp_amd
+    from culamp_amd import kernels as K
+    n, m = len(x), len(y)
+    dx = culamp_amd.DeviceArray.from_host(ctx, x)
+    dy = culamp_amd.DeviceArray.from_host(ctx, y)
+    dth = culamp_amd.DeviceArray.from_host(ctx, theta)
+    dout = culamp_amd.DeviceArray(ctx, n)
+    reg = a.eta / 4.0  # eta_t at convergence = eta_p / (2 W_inf), W_inf = 2 on the unit grid
+    for _ in range(5):
+        K.warp_logsumexp_spp_ct_fused(ctx, dout, dx, dy, dth, reg, 1.0, 2.0, 2.0, culamp_amd.MIXED)
+    ctx.timer_start()
+    for _ in range(a.kernel_launches):
+        K.warp_logsumexp_spp_ct_fused(ctx, dout, dx, dy, dth, reg, 1.0, 2.0, 2.0, culamp_amd.MIXED)
+    ms = ctx.timer_stop() / a.kernel_launches
+    bpl = pass_bytes(n, m, 4)
+    achieved = bpl / (ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    log(f"row-LSE kernel: {ms * 1e3:.2f} us/launch, {achieved:.0f} GB/s dense-eq")
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "pair_pass<mixed, LSE, sqL2> + pass_merge (culamp_lse_rows)",
+            "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": bpl,
+            "pairs_per_s": n * m / (ms * 1e-3)}
+
+
+def kernel_only(a) -> None:
+    """Launches only the roofline kernel (and, for counter calibration, a
+    device-to-device copy of 64 MiB) so that a profiler run sees them alone."""
+    import numpy as np
+    import culamp_amd
+    from culamp_amd import workloads
+
+    def log(*x):
+        print(*x, file=sys.stderr, flush=True)
+
+    ctx = culamp_amd.Context(0)
+    x, y, r, c = workloads.grid_histograms(a.grid)
+    theta = np.random.default_rng(0).uniform(-1e-3, 1e-3, len(y))
+    lse_roofline(ctx, x, y, theta, a, log)
+    nbytes = 64 << 20
+    src, dst = culamp_amd.DeviceArray(ctx, nbytes // 8), culamp_amd.DeviceArray(ctx, nbytes // 8)
+    for _ in range(10):
+        culamp_amd._lib.check(ctx.lib.culamp_memcpy_dtod(ctx.handle, dst.ptr, src.ptr, nbytes))
+    ctx.synchronize()
+    log(f"copy calibration: 10 x {nbytes} B device-to-device")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,7 +186,12 @@ def main() -> None:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--plain-seconds", type=float, default=60.0,
                     help="time budget of the sequential FP64 single-instance comparison")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="launch only the roofline kernel and a copy of known size (profiling)")
     a = ap.parse_args()
+    if a.kernel_only:
+        kernel_only(a)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
               "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
@@ -186,27 +247,7 @@ def main() -> None:
         }
         log(f"fp64 plain: {d64:.3f}s, {i64['iterations']} it, infeas {i64['residual']:.3e}")
 
-    # ---- roofline of the dominant kernel: mixed row LSE, events on the library stream
-    from culamp_amd import kernels as K
-    dx = culamp_amd.DeviceArray.from_host(ctx, x)
-    dy = culamp_amd.DeviceArray.from_host(ctx, y)
-    dth = culamp_amd.DeviceArray.from_host(ctx, nu)
-    dout = culamp_amd.DeviceArray(ctx, n)
-    reg = a.eta / 4.0  # eta_t at convergence = eta_p / (2 W_inf), W_inf = 2 on the unit grid
-    for _ in range(5):
-        K.warp_logsumexp_spp_ct_fused(ctx, dout, dx, dy, dth, reg, 1.0, 2.0, 2.0, culamp_amd.MIXED)
-    ctx.timer_start()
-    for _ in range(a.kernel_launches):
-        K.warp_logsumexp_spp_ct_fused(ctx, dout, dx, dy, dth, reg, 1.0, 2.0, 2.0, culamp_amd.MIXED)
-    ms = ctx.timer_stop() / a.kernel_launches
-    bpl = pass_bytes(n, m, 4)
-    achieved = bpl / (ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "pair_pass<mixed, LSE, sqL2> (culamp_lse_rows)",
-                "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": bpl,
-                "pairs_per_s": n * m / (ms * 1e-3)}
-    log(f"row-LSE kernel: {ms * 1e3:.2f} us/launch, {achieved:.0f} GB/s dense-eq")
+    roofline = lse_roofline(ctx, x, y, nu, a, log)
 
     cpu = None
     if not a.no_cpu:

@@ -535,6 +535,13 @@ int culamp_memcpy_dtoh(culamp_ctx* ctx, void* dst, const void* src, size_t bytes
   return CULAMP_OK;
 }
 
+int culamp_memcpy_dtod(culamp_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  TRY(check_ctx(ctx));
+  if (bytes == 0) return CULAMP_OK;
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return CULAMP_OK;
+}
+
 int culamp_timer_start(culamp_ctx* ctx) {
   TRY(check_ctx(ctx));
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));

@@ -90,6 +90,7 @@ struct TermState {
   int ring = 0, every = 1;
   std::vector<int64_t> ck_iter;
   int64_t T = 0;
+  int64_t t_mixed = INT64_MAX;  // iterations after this one ran in FP64 (polish)
 };
 
 int store_ck(culamp_ctx* ctx, const Term& tm, TermState& S, int64_t t) {
@@ -145,20 +146,28 @@ int forward(culamp_ctx* ctx, const Term& tm, double eps, const culamp_divergence
   int64_t t = 0;
   int converged = 0;
   double resid = NAN;
+  int prec = o.precision;
   auto residual = [&]() -> int {
-    TRY(run_pass(ctx, d3, o.precision, KIND_SUM));
+    TRY(run_pass(ctx, d3, prec, KIND_SUM));
     ReduceSpec rs{};
     rs.nops = 1; rs.op[0] = 5; rs.a[0] = S.res; rs.len[0] = tm.n;
     return reduce_to_host(ctx, rs, &resid);
   };
   while (t < o.itermax) {
-    TRY(run_pass(ctx, h.d1, o.precision, KIND_LSE));
-    TRY(run_pass(ctx, h.d2, o.precision, KIND_LSE));
+    TRY(run_pass(ctx, h.d1, prec, KIND_LSE));
+    TRY(run_pass(ctx, h.d2, prec, KIND_LSE));
     ++t;
     if (o.unroll > 0 && t % S.every == 0) TRY(store_ck(ctx, tm, S, t));
     if (t % o.frequency == 0) {
       TRY(residual());
-      if (resid <= o.tol) {
+      if (resid <= (prec == o.precision ? o.tol : o.polish_tol)) {
+        if (prec != CULAMP_FP64 && o.polish_tol > 0.0) {
+          // continue in FP64 from the mixed solution to the polish tolerance
+          S.t_mixed = t;
+          prec = CULAMP_FP64;
+          TRY(residual());
+          if (resid > o.polish_tol) continue;
+        }
         converged = 1;
         break;
       }
@@ -255,8 +264,9 @@ int backward(culamp_ctx* ctx, const Term& tm, double eps, const culamp_divergenc
     HIP_TRY(hipMemcpyAsync(S.phi, ck_phi, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(S.psi, ck_psi, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
     for (int64_t t = start + 1; t <= end; ++t) {
-      TRY(run_pass(ctx, h.d1, o.precision, KIND_LSE));
-      TRY(run_pass(ctx, h.d2, o.precision, KIND_LSE));
+      const int pr = t <= S.t_mixed ? o.precision : CULAMP_FP64;
+      TRY(run_pass(ctx, h.d1, pr, KIND_LSE));
+      TRY(run_pass(ctx, h.d2, pr, KIND_LSE));
       const int64_t q = t - start - 1;
       HIP_TRY(hipMemcpyAsync(seg_phi + q * n, S.phi, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
       HIP_TRY(hipMemcpyAsync(seg_psi + q * m, S.psi, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
@@ -321,6 +331,7 @@ struct SymState {
   int ring = 0, every = 1;
   std::vector<int64_t> ck_iter;
   int64_t T = 0;
+  int64_t t_mixed = INT64_MAX;  // iterations after this one ran in FP64 (polish)
 };
 
 int sym_store(culamp_ctx* ctx, int64_t n, SymState& S, int64_t t) {
@@ -363,19 +374,27 @@ int sym_forward(culamp_ctx* ctx, const double* x, const double* a, int64_t n, do
   int64_t t = 0;
   int converged = 0;
   double resid = NAN;
+  int prec = o.precision;
   auto residual = [&]() -> int {
-    TRY(run_pass(ctx, d3, o.precision, KIND_SUM));
+    TRY(run_pass(ctx, d3, prec, KIND_SUM));
     ReduceSpec rs{};
     rs.nops = 1; rs.op[0] = 5; rs.a[0] = S.res; rs.len[0] = n;
     return reduce_to_host(ctx, rs, &resid);
   };
   while (t < o.itermax) {
-    TRY(sym_step(ctx, x, a, n, eps, o.precision, S));
+    TRY(sym_step(ctx, x, a, n, eps, prec, S));
     ++t;
     if (o.unroll > 0 && t % S.every == 0) TRY(sym_store(ctx, n, S, t));
     if (t % o.frequency == 0) {
       TRY(residual());
-      if (resid <= o.tol) {
+      if (resid <= (prec == o.precision ? o.tol : o.polish_tol)) {
+        if (prec != CULAMP_FP64 && o.polish_tol > 0.0) {
+          // continue in FP64 from the mixed solution to the polish tolerance
+          S.t_mixed = t;
+          prec = CULAMP_FP64;
+          TRY(residual());
+          if (resid > o.polish_tol) continue;
+        }
         converged = 1;
         break;
       }
@@ -448,7 +467,7 @@ int sym_backward(culamp_ctx* ctx, const double* x, const double* a, int64_t n, d
     HIP_TRY(hipMemcpyAsync(S.Tf, S.ck_Tf + (int64_t)slot * n, sizeof(double) * n,
                            hipMemcpyDeviceToDevice, s));
     for (int64_t t = start + 1; t <= end; ++t) {
-      TRY(sym_step(ctx, x, a, n, eps, o.precision, S));
+      TRY(sym_step(ctx, x, a, n, eps, t <= S.t_mixed ? o.precision : CULAMP_FP64, S));
       const int64_t q = t - start - 1;
       HIP_TRY(hipMemcpyAsync(seg_f + q * n, S.f, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
       HIP_TRY(hipMemcpyAsync(seg_Tf + q * n, S.Tf, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -489,6 +508,7 @@ int check_opts(const culamp_divergence_opts* o) {
   if (o->frequency <= 0 || o->itermax <= 0)
     return fail(CULAMP_ERR_INVALID_ARG, "frequency and itermax must be positive");
   if (o->unroll < 0 || o->checkpoint < 0) return fail(CULAMP_ERR_INVALID_ARG, "negative unroll / checkpoint");
+  if (o->polish_tol < 0.0) return fail(CULAMP_ERR_INVALID_ARG, "polish_tol must be >= 0");
   if (o->precision != CULAMP_FP64 && o->precision != CULAMP_MIXED)
     return fail(CULAMP_ERR_INVALID_ARG, "precision must be CULAMP_FP64 or CULAMP_MIXED");
   return CULAMP_OK;
@@ -587,6 +607,7 @@ void culamp_divergence_opts_default(culamp_divergence_opts* o) {
   o->unroll = 64;
   o->checkpoint = 16;
   o->precision = CULAMP_MIXED;
+  o->polish_tol = 1e-9;
 }
 
 int culamp_sinkhorn_divergence(culamp_ctx* ctx, const double* x, int64_t n, const double* a,

@@ -140,18 +140,18 @@ struct BatchLaunch {
 // Apply the dynamic scalars of a pass (see PassDesc::dyn).
 __device__ __forceinline__ PassDesc resolve_desc(const PassDesc& din) {
 #pragma clang fp contract(off)
+  // (selects rather than a field chosen at run time, so that the copy stays
+  // in registers instead of private memory)
   PassDesc d = din;
   if (d.dyn_kind != DYN_NONE) {
-    const double reg = d.dyn[0], st = d.dyn[1];
+    const double reg = d.dyn[0], st = d.dyn[1], mix = d.dyn[2];
     const double c1 = st / (2.0 * d.dyn_winf);
     const double invreg = 1.0 / reg;
+    const bool lse = d.dyn_kind == DYN_LSE;
     d.k = -c1 * invreg;
-    if (d.dyn_kind == DYN_LSE) {
-      d.alpha = -invreg;
-    } else {
-      d.gamma = -invreg;
-      d.mix_t = d.dyn[2];
-    }
+    d.alpha = lse ? -invreg : d.alpha;
+    d.gamma = lse ? d.gamma : -invreg;
+    d.mix_t = lse ? d.mix_t : mix;
   }
   return d;
 }

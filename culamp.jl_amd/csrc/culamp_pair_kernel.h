@@ -345,6 +345,52 @@ __device__ __forceinline__ void source_terms(const PassDesc& d, int64_t s, R* a_
   }
 }
 
+// Mixed-precision LSE accumulation with a lazily moved reference.  The
+// caller forms each exponent relative to the target's reference as
+// v' = k C + (a_s + tqm), tqm = fl(tq - m), with m integer valued; S holds
+// sum 2^(v_true + tq - m + dl), dl = tqm - (tq - m) exactly (FP64).  While the
+// sub-tile sum stays in [2^-60, 2^100] relative to the reference (the common
+// case once the target offset centres the exponents near 0) no per-pair max
+// or shift is needed; otherwise the reference moves to the sub-tile max, S is
+// rescaled exactly in FP64 and the sub-tile is re-evaluated.  The caller
+// removes dl from S at the end (S *= 2^-dl), which restores the (m, S)
+// convention of the other kinds.
+template <int NV>
+__device__ __forceinline__ void lse_lazy(TState<float>& st, const float (&v)[NV],
+                                         const float (&wv)[NV], const float tq, float& tqm,
+                                         double& dl) {
+  float loc = 0.0f;
+#pragma unroll
+  for (int u = 0; u < NV; ++u) loc = fmaf(exp2_fast(v[u]), wv[u], loc);
+  const double Sn = st.S + (double)loc;
+  if (loc < 0x1p100f && Sn > 0x1p-60) {
+    st.S = Sn;
+    return;
+  }
+  float tm = v[0];
+#pragma unroll
+  for (int u = 1; u < NV; ++u) tm = fmaxf(tm, v[u]);
+  // new reference: above both the sub-tile's terms and what S already holds
+  const float top = fmaxf(tm, st.S > 0.0 ? (float)log2(st.S) : -INFINITY);
+  if (!(top > -INFINITY) || !(top < INFINITY)) {
+    st.S = Sn;
+    return;
+  }
+  const float c = ceilf(top);
+  const float mnew = st.m + c;
+  const float tqm_new = tq - mnew;
+  const double dnew = (double)tqm_new - ((double)tq - (double)mnew);
+  const float dv = tqm_new - tqm;
+  st.S = ldexp(st.S, -(int)c) * exp2(dnew - dl);
+  loc = 0.0f;
+#pragma unroll
+  for (int u = 0; u < NV; ++u) loc = fmaf(exp2_fast(v[u] + dv), wv[u], loc);
+  st.S += (double)loc;
+  st.m = mnew;
+  tqm = tqm_new;
+  dl = dnew;
+}
+
 // Fold NV exponents of one target into its running state.
 template <int PREC, int KIND, int NV, typename R>
 __device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], const R (&wv)[NV],
@@ -353,7 +399,7 @@ __device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], cons
   if constexpr (KIND == KIND_LSE || KIND == KIND_BARY) {
     R tm = v[0];
 #pragma unroll
-    for (int u = 1; u < NV; ++u) tm = tm > v[u] ? tm : v[u];
+    for (int u = 1; u < NV; ++u) tm = fmax(tm, v[u]);
     if constexpr (PREC == 1) {
       const float mn = fmaxf(st.m, ceilf(tm));
       const int sh = (int)fmaxf(st.m - mn, -2000.0f);
@@ -417,7 +463,10 @@ __device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, cons
   const R k = PREC == 0 ? (R)d.k : (R)(d.k * kLog2e);
   const R pexp = (R)d.p;
 
-  R tx[RPT], ty[RPT], tz[RPT], tq[RPT];
+  // mixed LSE: lazily moved reference (see lse_lazy)
+  constexpr bool kLazy = (PREC == 1 && KIND == KIND_LSE);
+  R tx[RPT], ty[RPT], tz[RPT], tq[RPT], tqm[RPT];
+  double dl[RPT];
   int64_t tidx[RPT];
   TState<R> st[RPT];
 #pragma unroll
@@ -437,9 +486,11 @@ __device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, cons
       tq[r] = (R)q;
       st[r].qd = q;
     }
-    st[r].m = empty_max<PREC, R>();
+    st[r].m = kLazy ? (R)0 : empty_max<PREC, R>();
     st[r].S = st[r].Sc = st[r].Sy0 = st[r].Sy1 = st[r].Sy2 = 0.0;
     st[r].acc = KIND == KIND_MIN ? (R)INFINITY : (R)-INFINITY;
+    tqm[r] = tq[r];
+    dl[r] = 0.0;
   }
 
   for (int64_t base = s0; base < s1; base += kTile) {
@@ -490,17 +541,20 @@ __device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, cons
           for (int r = 0; r < RPT; ++r) {
             const R C = pair_cost<COST>(tx[r], ty[r], tz[r], sx, sy, sz, pexp);
             Cv[r][u] = C;
-            v[r][u] = fma(k, C, sa + tq[r]);
+            v[r][u] = fma(k, C, sa + (kLazy ? tqm[r] : tq[r]));
           }
         }
 #pragma unroll
-        for (int r = 0; r < RPT; ++r)
-          accumulate<PREC, KIND, kSub, R>(st[r], v[r], wv, Cv[r], px, py, pz);
+        for (int r = 0; r < RPT; ++r) {
+          if constexpr (kLazy) lse_lazy<kSub>(st[r], v[r], wv, tq[r], tqm[r], dl[r]);
+          else accumulate<PREC, KIND, kSub, R>(st[r], v[r], wv, Cv[r], px, py, pz);
+        }
       }
     }
   }
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
+    if constexpr (kLazy) st[r].S *= exp2(-dl[r]);
     combine_lanes<PREC, KIND, G, R>(st[r]);
     if (g != 0) tidx[r] = ntp;  // one owner lane per target
   }

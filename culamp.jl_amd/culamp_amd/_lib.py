@@ -75,8 +75,9 @@ class culamp_batch_tuning(C.Structure):
 
 class culamp_divergence_opts(C.Structure):
     _fields_ = [("eps", C.c_double), ("eta_p", C.c_double), ("tol", C.c_double),
-                ("itermax", C.c_int64), ("frequency", C.c_int64), ("unroll", C.c_int32),
-                ("checkpoint", C.c_int32), ("precision", C.c_int32), ("reserved", C.c_int32)]
+                ("polish_tol", C.c_double), ("itermax", C.c_int64), ("frequency", C.c_int64),
+                ("unroll", C.c_int32), ("checkpoint", C.c_int32), ("precision", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 LOG_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)
@@ -100,6 +101,7 @@ SIGNATURES = {
     "culamp_free": (_I, [_P, _P]),
     "culamp_memcpy_htod": (_I, [_P, _P, _P, C.c_size_t]),
     "culamp_memcpy_dtoh": (_I, [_P, _P, _P, C.c_size_t]),
+    "culamp_memcpy_dtod": (_I, [_P, _P, _P, C.c_size_t]),
     "culamp_timer_start": (_I, [_P]),
     "culamp_timer_stop": (_I, [_P, C.POINTER(_D)]),
     "culamp_max_cost": (_I, [_P, _P, _I64, _P, _I64, _D, _P]),

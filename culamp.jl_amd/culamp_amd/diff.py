@@ -21,7 +21,8 @@ from .solvers import default_context
 
 def divergence_opts(eps: float | None = None, eta_p: float = 1e-2, tol: float = 1e-6,
                     itermax: int = 100000, frequency: int = 10, unroll: int = 64,
-                    checkpoint: int = 16, precision: str = "mixed") -> culamp_divergence_opts:
+                    checkpoint: int = 16, precision: str = "mixed",
+                    polish_tol: float = 1e-9) -> culamp_divergence_opts:
     o = culamp_divergence_opts()
     load().culamp_divergence_opts_default(C.byref(o))
     o.eps = 0.0 if eps is None else float(eps)
@@ -32,6 +33,7 @@ def divergence_opts(eps: float | None = None, eta_p: float = 1e-2, tol: float = 
     o.unroll = int(unroll)
     o.checkpoint = int(checkpoint)
     o.precision = {"fp64": FP64, "mixed": MIXED}[precision]
+    o.polish_tol = float(polish_tol)
     return o
 
 

@@ -123,6 +123,7 @@ int culamp_malloc(culamp_ctx *ctx, size_t bytes, void **dptr);
 int culamp_free(culamp_ctx *ctx, void *dptr);
 int culamp_memcpy_htod(culamp_ctx *ctx, void *dst, const void *src, size_t bytes);
 int culamp_memcpy_dtoh(culamp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int culamp_memcpy_dtod(culamp_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* HIP-event timer on the context's stream: start, then stop returns the
  * elapsed milliseconds between the two events (waits for the stop event). */
 int culamp_timer_start(culamp_ctx *ctx);
@@ -371,6 +372,9 @@ typedef struct {
   double eps;         /* > 0: regularisation; <= 0: eta_p * max_ij C(x_i, y_j) */
   double eta_p;       /* used when eps <= 0 */
   double tol;         /* stop: L1 row infeasibility <= tol */
+  double polish_tol;  /* mixed forward: once converged, continue in FP64 until the FP64
+                         infeasibility is <= polish_tol (0: no FP64 stage); the FP32
+                         rounding of the potentials otherwise limits the weight gradient */
   int64_t itermax;
   int64_t frequency;  /* infeasibility tested every `frequency` iterations */
   int32_t unroll;     /* iterations back-propagated at the end of each solve (0: envelope only) */
