@@ -47,6 +47,9 @@ struct culamp_batch {
   int64_t sum_n = 0, sum_m = 0;
   double *r = nullptr, *c = nullptr, *logr = nullptr;  // pooled marginals
   culamp_batch_tuning tune{};
+  // W_inf of every instance per cost exponent p (feature batches: one entry),
+  // computed by the first solve that needs it: it depends on the data only
+  std::vector<std::pair<double, std::vector<double>>> winf_cache;
 };
 
 namespace {
@@ -258,7 +261,7 @@ struct Engine {
       for (int32_t sp = 0; sp < bi.splits; ++sp)
         for (int64_t tb = 0; tb < tblocks; ++tb) P.items.push_back(BatchItem{k, sp, tb});
       if (bi.splits > 1) {
-        const int64_t per = kThreads / 4;
+        const int64_t per = kThreads / kMergeLanes;
         const int64_t mb = (nt_of(k, P.rows) + per - 1) / per;
         for (int64_t t = 0; t < mb; ++t) P.mitems.push_back(BatchItem{k, 0, t});
       }
@@ -428,6 +431,11 @@ int upload_descs(DevArr<PassDesc>& dst, const std::vector<PassDesc>& v, hipStrea
 int batch_winf(Engine& E, double p, double* tmp_n, std::vector<double>& winf) {
   culamp_batch* b = E.b;
   culamp_ctx* ctx = E.ctx;
+  for (const auto& e : b->winf_cache)
+    if (e.first == p || (std::isnan(e.first) && std::isnan(p))) {
+      winf = e.second;
+      return CULAMP_OK;
+    }
   std::vector<PassDesc> dm(b->count);
   std::vector<ReduceSpec> rs(b->count);
   for (int32_t k = 0; k < b->count; ++k) {
@@ -453,6 +461,7 @@ int batch_winf(Engine& E, double p, double* tmp_n, std::vector<double>& winf) {
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   winf.resize(b->count);
   for (int32_t k = 0; k < b->count; ++k) winf[k] = b->feature ? std::fmax(h[8 * k], 0.0) : h[8 * k];
+  b->winf_cache.emplace_back(p, winf);
   return CULAMP_OK;
 }
 
@@ -665,7 +674,7 @@ int batch_lamp(culamp_ctx* ctx, culamp_batch* b, const culamp_args* a, int64_t f
 
   // descriptor tables: check passes E1 (LSE of nu), E2 (column pass), E3
   // (dual); iteration passes A..D
-  std::vector<PassDesc> hE1(K), hE2(K), hE3(K), hA(K), hB(K), hC(K), hD(K);
+  std::vector<PassDesc> hE1(K), hE2(K), hE3(K), hA(K), hB(K), hC(K), hD(K), hF(K);
   std::vector<ReduceSpec> hR1(K), hR2(K);
   for (int32_t k = 0; k < K; ++k) {
     culamp_problem* pb = b->pbs[k];
@@ -712,9 +721,12 @@ int batch_lamp(culamp_ctx* ctx, culamp_batch* b, const culamp_args* a, int64_t f
     hA[k] = lse(nu + om, logZ + on, dk + 4);
     hB[k] = col(nu + om, dk + 8);
     fuse(hB[k], 0, thetab + om, theta + om, nub + om);
+    hB[k].out1 = nullptr;  // the iteration passes need no column costs
     hC[k] = lse(nub + om, logZ + on, dk + 12);
     hD[k] = col(nub + om, dk + 16);
+    hF[k] = hD[k];  // cellsim: the last iteration's column costs, recomputed at the end
     fuse(hD[k], 1, theta + om, thetab + om, nu + om);
+    hD[k].out1 = nullptr;
     ReduceSpec r1{};
     r1.nops = 4;
     r1.op[0] = 1; r1.a[0] = res + om; r1.b[0] = ck; r1.len[0] = m;
@@ -733,12 +745,13 @@ int batch_lamp(culamp_ctx* ctx, culamp_batch* b, const culamp_args* a, int64_t f
     r2.op[1] = 2; r2.a[1] = ck; r2.b[1] = (cellsim && !(eta[k] > 0)) ? gam + om : theta + om; r2.len[1] = m;
     hR2[k] = r2;
   }
-  DevArr<PassDesc> DE1, DE2, DE3, DA, DB, DC, DD;
+  DevArr<PassDesc> DE1, DE2, DE3, DA, DB, DC, DD, DF;
   DevArr<ReduceSpec> DR1, DR2;
   TRY(upload_descs(DE1, hE1, ctx->stream)); TRY(upload_descs(DE2, hE2, ctx->stream));
   TRY(upload_descs(DE3, hE3, ctx->stream)); TRY(upload_descs(DA, hA, ctx->stream));
   TRY(upload_descs(DB, hB, ctx->stream)); TRY(upload_descs(DC, hC, ctx->stream));
   TRY(upload_descs(DD, hD, ctx->stream));
+  TRY(upload_descs(DF, hF, ctx->stream));
   TRY(DR1.upload(hR1, ctx->stream)); TRY(DR2.upload(hR2, ctx->stream));
   // the dual pass: LSE when eta > 0 (every instance of a batch shares the
   // sign of eta_p), min-plus (FP64) otherwise
@@ -855,7 +868,18 @@ int batch_lamp(culamp_ctx* ctx, culamp_batch* b, const culamp_args* a, int64_t f
     TRY(E.launch(DE1.p, prec, KIND_LSE, true, pp));
     for (int32_t k = 0; k < K; ++k) npass[k] += 1;
   } else {
-    // objective = sum of the last column-pass costs (MirrorProxCellSim.jl:184)
+    // objective = sum of the last column-pass costs (MirrorProxCellSim.jl:184):
+    // instances stopped by a check have them from the check's column pass,
+    // the others recompute the last iteration's column pass with costs
+    std::vector<int32_t> unconv;
+    for (int32_t k = 0; k < K; ++k)
+      if (!conv[k]) unconv.push_back(k);
+    if (!unconv.empty()) {
+      TRY(E.set_active(unconv));
+      TRY(E.launch(DF.p, prec, KIND_SUM, false, pp));
+      for (int32_t k : unconv) npass[k] += 1;
+      TRY(E.set_active(all_instances(b)));
+    }
     std::vector<ReduceSpec> ro(K);
     for (int32_t k = 0; k < K; ++k) {
       ReduceSpec r{};

@@ -166,6 +166,7 @@ struct LaunchCfg {
 constexpr int kThreads = 256;
 constexpr int kTile = 256;      // sources staged in LDS per step
 constexpr int kLanesPerTarget = 4;  // point kernel: adjacent lanes sharing a target
+constexpr int kMergeLanes = 16;     // split merge: adjacent lanes sharing a target
 constexpr int kMaxFields = 5;   // partial fields per target (BARY)
 
 int pass_fields(int kind);

@@ -254,10 +254,11 @@ __device__ __forceinline__ double target_offset(const PassDesc& d, int64_t t) {
   return q;
 }
 
-// Combine the split partials of each target, then apply the epilogue.  Four
-// adjacent lanes share a target and read interleaved splits (coalesced over
-// targets); they combine by butterfly, so the result is deterministic.
-constexpr int kMergeLanes = 4;
+// Combine the split partials of each target, then apply the epilogue.
+// kMergeLanes adjacent lanes share a target and read interleaved splits (one
+// split each up to kMergeLanes splits, so the loads are one round trip); they
+// combine by butterfly, so the result is deterministic.
+
 template <int PREC, int KIND>
 __device__ __forceinline__ void merge_body(const PassDesc& d, const Work& w, const int64_t block,
                                            const int splits, const int64_t ntp) {
@@ -392,7 +393,7 @@ __device__ __forceinline__ void lse_lazy(TState<float>& st, const float (&v)[NV]
 }
 
 // Fold NV exponents of one target into its running state.
-template <int PREC, int KIND, int NV, typename R>
+template <int PREC, int KIND, int NV, typename R, bool NEEDC = true>
 __device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], const R (&wv)[NV],
                                            const R (&Cv)[NV], const R (&px)[NV],
                                            const R (&py)[NV], const R (&pz)[NV]) {
@@ -439,7 +440,7 @@ __device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], cons
       if constexpr (PREC == 1) e = exp2_fast(v[u]) * wv[u];
       else e = exp(v[u]) * wv[u];
       ls += e;
-      lc = fma(e, Cv[u], lc);
+      if constexpr (NEEDC) lc = fma(e, Cv[u], lc);
     }
     st.S += (double)ls;
     st.Sc += (double)lc;
@@ -880,7 +881,8 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
   constexpr int TILE = 32;
   __shared__ __bf16 Ah[TILE * LDW];
   __shared__ __bf16 Al[NSPLIT == 3 ? TILE * LDW : 1];
-  __shared__ float s_a[TILE], s_w[TILE];
+  __shared__ __align__(16) float s_a[TILE];
+  __shared__ __align__(16) float s_w[TILE];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int64_t s0 = (int64_t)split * split_len;
@@ -918,11 +920,13 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
     const double q = tok ? target_offset<1, KIND>(d, t) : 0.0;
     const float q2 = (float)(q * kLog2e);
     st.qd = (double)q2;
-    st.m = empty_max<1, float>();
+    st.m = KIND == KIND_LSE ? 0.0f : empty_max<1, float>();
     st.S = st.Sc = st.Sy0 = st.Sy1 = st.Sy2 = 0.0;
     st.acc = 0.0f;
   }
   const float tq = (float)st.qd;
+  float tqm = tq;    // LSE: exponent offset relative to the lazily moved reference
+  double dl = 0.0;
 
   for (int64_t base = s0; base < s1; base += TILE) {
     const int cnt = (int)min((int64_t)TILE, s1 - base);
@@ -971,23 +975,52 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[ks], acc, 0, 0, 0);
       }
     }
+    // the lane's 16 sources are 4 runs of 4 consecutive rows: i = 8 q + 4 (lane / 32) + e;
+    // padded sources carry a_s = -inf, w_s = 0
+    float sa[16], sw[16];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      float v[8], wv[8], Cv[8], dz[8];
+    for (int q = 0; q < 4; ++q) {
+      const int i0 = 8 * q + 4 * (lane >> 5);
+      const float4 a4 = *(const float4*)(s_a + i0);
+      const float4 w4 = *(const float4*)(s_w + i0);
+      sa[4 * q] = a4.x; sa[4 * q + 1] = a4.y; sa[4 * q + 2] = a4.z; sa[4 * q + 3] = a4.w;
+      sw[4 * q] = w4.x; sw[4 * q + 1] = w4.y; sw[4 * q + 2] = w4.z; sw[4 * q + 3] = w4.w;
+    }
+    if constexpr (KIND == KIND_LSE) {
+      float v[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int r = h * 8 + u;
-        const int i = (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
-        float sim = acc[r];
-        sim = sim < -1.0f ? -1.0f : (sim > 1.0f ? 1.0f : sim);
-        Cv[u] = (1.0f - sim) * inv_scale;
-        wv[u] = s_w[i];
-        dz[u] = 0.0f;
-        v[u] = i < cnt ? fmaf(kc, sim, s_a[i] + tq) : -INFINITY;
+      for (int r = 0; r < 16; ++r) {
+        const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
+        v[r] = fmaf(kc, sim, sa[r] + tqm);
       }
-      accumulate<1, KIND, 8, float>(st, v, wv, Cv, dz, dz, dz);
+      lse_lazy<16>(st, v, sw, tq, tqm, dl);
+    } else if (d.out1 != nullptr) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        float v[8], wv[8], Cv[8], dz[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int r = h * 8 + u;
+          const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
+          Cv[u] = (1.0f - sim) * inv_scale;
+          wv[u] = sw[r];
+          dz[u] = 0.0f;
+          v[u] = fmaf(kc, sim, sa[r] + tq);
+        }
+        accumulate<1, KIND, 8, float>(st, v, wv, Cv, dz, dz, dz);
+      }
+    } else {  // no cost output: marginal only
+      float v[16], dz[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
+        dz[r] = 0.0f;
+        v[r] = fmaf(kc, sim, sa[r] + tq);
+      }
+      accumulate<1, KIND, 16, float, false>(st, v, sw, dz, dz, dz, dz);
     }
   }
+  if constexpr (KIND == KIND_LSE) st.S *= exp2(-dl);
   // merge the two half-waves holding the same target
   if constexpr (KIND == KIND_LSE) {
     const float mo = __shfl_xor(st.m, 32);
