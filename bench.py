@@ -17,8 +17,10 @@ Printed as one JSON line (rank 0):
   cpu_baseline the FP64 oracle LAMP (restatement of the reference loop) on a
                bounded sample of the same workload on the host cores.
 Extra fields: solves/s and iterations/s for mixed and for the plain FP64
-single-instance path (host-checked stop every `frequency` iterations), and
-the mixed-vs-FP64 error of the transport cost.
+single-instance path (host-checked stop every `frequency` iterations), the
+mixed-vs-FP64 error of the transport cost, `batched` (256 cell-similarity
+instances in one batch vs the plain FP64 path one instance at a time) and
+`divergence` (Sinkhorn divergence with gradients, 65536 points).
 """
 from __future__ import annotations
 
@@ -116,6 +118,39 @@ def bench_batched(ctx, a, log) -> dict:
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_lse_mixed_grid64.json")
 
 
+def bench_divergence(ctx, a, log) -> dict:
+    """Differentiable Sinkhorn divergence: x ~ N(0, I_3) (n points), y a
+    4-component Gaussian mixture, Uniform(0.5, 1.5) weights; squared
+    Euclidean cost, eps = eta_p * max C (eta_p = 1e-2); forward to L1
+    infeasibility 1e-6 (mixed, then FP64 to 1e-9), backward = envelope +
+    adjoint of the last 64 iterations of each solve, checkpoints every 16."""
+    import numpy as np
+    import culamp_amd
+    from culamp_amd import workloads
+    x, y, wa, wb = workloads.divergence_clouds(a.diff_n, a.diff_n)
+    kw = dict(ctx=ctx, eta_p=1e-2, tol=1e-6, unroll=64, checkpoint=16, precision="mixed",
+              frequency=10)
+    t0 = time.perf_counter()
+    (val, gx, ga, gy, gb), terms = culamp_amd.sinkhorn_divergence(x, wa, y, wb, return_terms=True,
+                                                                  **kw)
+    dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    val0, terms0 = culamp_amd.sinkhorn_divergence(x, wa, y, wb, grad=False, return_terms=True, **kw)
+    dtf = time.perf_counter() - t1
+    pairs = sum(t["pairs"] for t in terms)
+    log(f"divergence n={a.diff_n}: value {val:.6e}, forward+backward {dt:.2f}s, forward {dtf:.2f}s")
+    return {"workload": f"Sinkhorn divergence, n = m = {a.diff_n}, x ~ N(0, I3), y 4-component GMM, "
+                        "squared Euclidean, eps = 1e-2 * max C, forward to infeasibility 1e-6 "
+                        "(mixed + FP64 polish to 1e-9), unrolled adjoint over 64 iterations, "
+                        "checkpoints every 16",
+            "value": val, "seconds_forward_backward": dt, "seconds_forward": dtf,
+            "eps": terms[0]["eta_t"],
+            "terms": [{k: t[k] for k in ("iterations", "residual", "objective", "converged")}
+                      for t in terms],
+            "forward_pairs": pairs, "grad_x_norm": float(np.linalg.norm(gx)),
+            "grad_a_centred_norm": float(np.linalg.norm(ga - ga.mean()))}
+
+
 def lse_roofline(ctx, x, y, theta, a, log) -> dict:
     """Roofline of the dominant kernel: the mixed row LSE (pair_pass + split
     merge, culamp_lse_rows) timed with HIP events on the library's stream."""
@@ -186,6 +221,8 @@ def main() -> None:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--plain-seconds", type=float, default=60.0,
                     help="time budget of the sequential FP64 single-instance comparison")
+    ap.add_argument("--diff-n", type=int, default=65536)
+    ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="launch only the roofline kernel and a copy of known size (profiling)")
     a = ap.parse_args()
@@ -273,6 +310,7 @@ def main() -> None:
         log(f"cpu oracle: {it_c} it in {dc:.2f}s")
 
     batched = None if a.no_batch else bench_batched(ctx, a, log)
+    divergence = None if a.no_diff else bench_divergence(ctx, a, log)
 
     ms_per_step = dt / a.steps * 1e3
     rec = {
@@ -291,6 +329,8 @@ def main() -> None:
     rec.update(out_fp64)
     if batched is not None:
         rec["batched"] = batched
+    if divergence is not None:
+        rec["divergence"] = divergence
     print(json.dumps(rec), flush=True)
 
 
