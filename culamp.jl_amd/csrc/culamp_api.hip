@@ -58,6 +58,11 @@ void emit(culamp_log_fn fn, void* user, const std::string& s) {
 
 int ensure_work(culamp_ctx* ctx, size_t partial_doubles) {
   if (partial_doubles > ctx->partial_cap) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(ctx->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return fail(CULAMP_ERR_INVALID_ARG,
+                  "call needs a larger work buffer while capturing: issue it once before capturing");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ctx->partial) HIP_TRY(hipFree(ctx->partial));
     ctx->partial = nullptr;
@@ -100,8 +105,8 @@ int run_pass(culamp_ctx* ctx, const PassDesc& d, int precision, int kind) {
     return CULAMP_OK;
   }
   LaunchCfg cfg = choose_launch(d.nt, d.ns, precision, kind, ctx->num_cu,
-                                ctx->force_rpt, ctx->force_splits);
-  if (cfg.splits > 1) {
+                                ctx->force_rpt, ctx->force_splits, ctx->layout);
+  if (cfg.splits > 1 && !cfg.wide) {
     const size_t ntp = (size_t)cfg.tblocks * (kThreads / kLanesPerTarget) * cfg.rpt;
     TRY(ensure_work(ctx, (size_t)pass_fields(kind) * cfg.splits * ntp));
   }
@@ -498,6 +503,14 @@ int culamp_ctx_set_launch(culamp_ctx* ctx, int rows_per_thread, int splits) {
   return CULAMP_OK;
 }
 
+int culamp_ctx_set_layout(culamp_ctx* ctx, int layout) {
+  TRY(check_ctx(ctx));
+  if (layout != LAYOUT_AUTO && layout != LAYOUT_SPLIT && layout != LAYOUT_WIDE)
+    return fail(CULAMP_ERR_INVALID_ARG, "layout must be 0 (auto), 1 (split) or 2 (wide)");
+  ctx->layout = layout;
+  return CULAMP_OK;
+}
+
 int culamp_ctx_synchronize(culamp_ctx* ctx) {
   TRY(check_ctx(ctx));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -556,6 +569,48 @@ int culamp_timer_stop(culamp_ctx* ctx, double* ms) {
   float f = 0.f;
   HIP_TRY(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
   *ms = (double)f;
+  return CULAMP_OK;
+}
+
+// ---- stream capture ---------------------------------------------------------
+struct culamp_graph {
+  hipGraphExec_t exec = nullptr;
+};
+
+int culamp_capture_begin(culamp_ctx* ctx) {
+  TRY(check_ctx(ctx));
+  HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  return CULAMP_OK;
+}
+
+int culamp_capture_end(culamp_ctx* ctx, culamp_graph** out) {
+  TRY(check_ctx(ctx));
+  if (!out) return fail(CULAMP_ERR_INVALID_ARG, "null output pointer");
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamEndCapture(ctx->stream, &g));
+  culamp_graph* gr = new culamp_graph();
+  const hipError_t e = hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    delete gr;
+    return fail(CULAMP_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  }
+  *out = gr;
+  return CULAMP_OK;
+}
+
+int culamp_graph_launch(culamp_ctx* ctx, culamp_graph* g) {
+  TRY(check_ctx(ctx));
+  if (!g) return fail(CULAMP_ERR_INVALID_ARG, "null graph");
+  HIP_TRY(hipGraphLaunch(g->exec, ctx->stream));
+  return CULAMP_OK;
+}
+
+int culamp_graph_destroy(culamp_graph* g) {
+  if (!g) return CULAMP_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  delete g;
   return CULAMP_OK;
 }
 

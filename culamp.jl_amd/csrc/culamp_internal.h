@@ -158,20 +158,30 @@ __device__ __forceinline__ PassDesc resolve_desc(const PassDesc& din) {
 
 struct LaunchCfg {
   int rpt;          // targets per thread
-  int splits;       // source splits (grid.y)
+  int splits;       // source splits (grid.y; inside the workgroup when wide)
   int64_t split_len;
   int64_t tblocks;  // grid.x
+  int wide;         // point passes: splits merged inside one workgroup (pair_pass_wide)
 };
+
+// Layout of a point pass's source splits: automatic, split over workgroups
+// (partials in HBM, merged by a second launch), or inside one workgroup.
+enum Layout : int { LAYOUT_AUTO = 0, LAYOUT_SPLIT = 1, LAYOUT_WIDE = 2 };
 
 constexpr int kThreads = 256;
 constexpr int kTile = 256;      // sources staged in LDS per step
 constexpr int kLanesPerTarget = 4;  // point kernel: adjacent lanes sharing a target
 constexpr int kMergeLanes = 16;     // split merge: adjacent lanes sharing a target
+// Workgroup-wide split layout (pair_pass_wide): one wave per source split,
+// one wave's worth of targets per workgroup.
+constexpr int kWideWaves = kMergeLanes;
+constexpr int kWideThreads = 64 * kWideWaves;
+constexpr int kWideTargets = 64 / kLanesPerTarget;
 constexpr int kMaxFields = 5;   // partial fields per target (BARY)
 
 int pass_fields(int kind);
 LaunchCfg choose_launch(int64_t nt, int64_t ns, int precision, int kind,
-                        int num_cu, int force_rpt, int force_splits);
+                        int num_cu, int force_rpt, int force_splits, int layout = LAYOUT_AUTO);
 hipError_t launch_pass(const PassDesc& d, int precision, int kind,
                        const LaunchCfg& cfg, const Work& w, hipStream_t s);
 hipError_t launch_pass_fp32(const PassDesc& d, int kind, const LaunchCfg& c,

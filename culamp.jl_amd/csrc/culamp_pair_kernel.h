@@ -346,15 +346,28 @@ __device__ __forceinline__ void source_terms(const PassDesc& d, int64_t s, R* a_
   }
 }
 
+// 2^x for the small arguments of lse_lazy's rescale (a difference of two
+// FP32 rounding errors, |x| <= 2^-4 unless an exponent offset exceeds 2^20
+// binades): Taylor polynomial of degree 6 in x ln 2, relative error below
+// 2^-36 for |x| <= 2^-4; larger arguments take the library exp2.
+__device__ __forceinline__ double exp2_small(double x) {
+  if (!(fabs(x) <= 0.0625)) return exp2(x);
+  const double y = x * kLn2;
+  return 1.0 + y * (1.0 + y * (0.5 + y * (1.0 / 6.0 + y * (1.0 / 24.0 + y * (1.0 / 120.0 +
+                                                                          y * (1.0 / 720.0))))));
+}
+
 // Mixed-precision LSE accumulation with a lazily moved reference.  The
 // caller forms each exponent relative to the target's reference as
 // v' = k C + (a_s + tqm), tqm = fl(tq - m), with m integer valued; S holds
 // sum 2^(v_true + tq - m + dl), dl = tqm - (tq - m) exactly (FP64).  While the
-// sub-tile sum stays in [2^-60, 2^100] relative to the reference (the common
+// sub-tile sum stays in [2^-90, 2^100] relative to the reference (the common
 // case once the target offset centres the exponents near 0) no per-pair max
-// or shift is needed; otherwise the reference moves to the sub-tile max, S is
-// rescaled exactly in FP64 and the sub-tile is re-evaluated.  The caller
-// removes dl from S at the end (S *= 2^-dl), which restores the (m, S)
+// or shift is needed: the FP32 terms of such a sum that could underflow are
+// below 2^-30 of it.  Otherwise the reference moves to the ceiling of the
+// larger of the sub-tile max and the binary exponent of S (an upper bound of
+// log2 S), S is rescaled in FP64 and the sub-tile is re-evaluated.  The
+// caller removes dl from S at the end (S *= 2^-dl), which restores the (m, S)
 // convention of the other kinds.
 template <int NV>
 __device__ __forceinline__ void lse_lazy(TState<float>& st, const float (&v)[NV],
@@ -364,15 +377,17 @@ __device__ __forceinline__ void lse_lazy(TState<float>& st, const float (&v)[NV]
 #pragma unroll
   for (int u = 0; u < NV; ++u) loc = fmaf(exp2_fast(v[u]), wv[u], loc);
   const double Sn = st.S + (double)loc;
-  if (loc < 0x1p100f && Sn > 0x1p-60) {
+  if (loc < 0x1p100f && Sn > 0x1p-90) {
     st.S = Sn;
     return;
   }
   float tm = v[0];
 #pragma unroll
   for (int u = 1; u < NV; ++u) tm = fmaxf(tm, v[u]);
-  // new reference: above both the sub-tile's terms and what S already holds
-  const float top = fmaxf(tm, st.S > 0.0 ? (float)log2(st.S) : -INFINITY);
+  int es = 0;
+  if (st.S > 0.0) (void)frexp(st.S, &es);
+  const float sl = st.S > 0.0 ? (float)es : -INFINITY;
+  const float top = fmaxf(tm, sl);
   if (!(top > -INFINITY) || !(top < INFINITY)) {
     st.S = Sn;
     return;
@@ -382,7 +397,7 @@ __device__ __forceinline__ void lse_lazy(TState<float>& st, const float (&v)[NV]
   const float tqm_new = tq - mnew;
   const double dnew = (double)tqm_new - ((double)tq - (double)mnew);
   const float dv = tqm_new - tqm;
-  st.S = ldexp(st.S, -(int)c) * exp2(dnew - dl);
+  st.S = ldexp(st.S, -(int)c) * exp2_small(dnew - dl);
   loc = 0.0f;
 #pragma unroll
   for (int u = 0; u < NV; ++u) loc = fmaf(exp2_fast(v[u] + dv), wv[u], loc);
@@ -447,37 +462,66 @@ __device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], cons
   }
 }
 
-template <int PREC, int KIND, int COST, int RPT>
-__device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, const int64_t tb,
-                                          const int split, const int splits,
-                                          const int64_t split_len, const int64_t ntp) {
-  using R = typename std::conditional<PREC == 0, double, float>::type;
+// A source point as staged in LDS: coordinates and exponent term in one
+// 16-byte (FP32) or 32-byte (FP64) record, so a lane reads it with one wide
+// LDS load; the weight sits in its own array.
+template <typename R>
+struct alignas(4 * sizeof(R)) SrcPt {
+  R x, y, z, a;
+};
+
+// Barrier among the NTHR threads that share a staged tile: the workgroup, or
+// one wave (a wave's LDS operations complete in issue order, so it needs only
+// compiler fences).
+template <int NTHR>
+__device__ __forceinline__ void group_sync() {
+  if constexpr (NTHR == kThreads) {
+    __syncthreads();
+  } else {
+    static_assert(NTHR == 64, "a group is a workgroup of kThreads or one wave");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// The pair loop of one group of NTHR threads (kLanesPerTarget adjacent lanes
+// per target, RPT targets per lane group) over sources [s0, s1), staged in
+// tiles of TILE through the group's LDS records s_p / s_w.  Targets are
+// t0 + r * (NTHR / kLanesPerTarget) + lane / kLanesPerTarget.  On return the
+// lanes have combined their states; the owner lane of each target keeps its
+// index in tidx, the others hold `none`.
+template <int PREC, int KIND, int COST, int RPT, int NTHR, int TILE, typename R>
+__device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* s_w,
+                                           const int tid, const int64_t t0, const int64_t s0,
+                                           const int64_t s1, const int64_t none,
+                                           int64_t (&tidx)[RPT], TState<R> (&st)[RPT]) {
   constexpr bool kExpKind = (KIND == KIND_LSE || KIND == KIND_SUM || KIND == KIND_BARY);
-  constexpr int G = kLanesPerTarget, TPB = kThreads / G;
+  constexpr int G = kLanesPerTarget, TPB = NTHR / G;
+  static_assert(TILE % (kSub * G) == 0, "tile / sub-tile shape");
 
-  __shared__ R s_x[kTile], s_y[kTile], s_z[kTile], s_a[kTile], s_w[kTile];
-
-  const int tid = threadIdx.x;
   const int g = tid % G, tt = tid / G;
-  const int64_t s0 = (int64_t)split * split_len;
-  const int64_t s1 = min(d.ns, s0 + split_len);
   const R k = PREC == 0 ? (R)d.k : (R)(d.k * kLog2e);
   const R pexp = (R)d.p;
 
   // mixed LSE: lazily moved reference (see lse_lazy)
   constexpr bool kLazy = (PREC == 1 && KIND == KIND_LSE);
+  // mixed squared-Euclidean LSE: coordinates are staged pre-multiplied by
+  // sqrt(-k) (k <= 0 in log2 units for every LSE pass), so the exponent
+  // (a_s + tqm) - |t' - s'|^2 is three fused multiply-adds after the
+  // differences, with no separate scaling of the cost
+  constexpr bool kScaled = kLazy && COST == COST_SQL2;
+  const double csc = kScaled ? sqrt(fmax(-(d.k * kLog2e), 0.0)) : 1.0;
   R tx[RPT], ty[RPT], tz[RPT], tq[RPT], tqm[RPT];
   double dl[RPT];
-  int64_t tidx[RPT];
-  TState<R> st[RPT];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    const int64_t t = tb * TPB * RPT + (int64_t)r * TPB + tt;
+    const int64_t t = t0 + (int64_t)r * TPB + tt;
     tidx[r] = t;
     const bool ok = t < d.nt;
-    tx[r] = ok ? (R)d.tgt[3 * t] : (R)0;
-    ty[r] = ok ? (R)d.tgt[3 * t + 1] : (R)0;
-    tz[r] = ok ? (R)d.tgt[3 * t + 2] : (R)0;
+    tx[r] = ok ? (R)(kScaled ? d.tgt[3 * t] * csc : d.tgt[3 * t]) : (R)0;
+    ty[r] = ok ? (R)(kScaled ? d.tgt[3 * t + 1] * csc : d.tgt[3 * t + 1]) : (R)0;
+    tz[r] = ok ? (R)(kScaled ? d.tgt[3 * t + 2] * csc : d.tgt[3 * t + 2]) : (R)0;
     const double q = ok ? target_offset<PREC, KIND>(d, t) : 0.0;
     if (PREC == 1) {
       const float q2 = (float)(q * kLog2e);
@@ -494,26 +538,32 @@ __device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, cons
     dl[r] = 0.0;
   }
 
-  for (int64_t base = s0; base < s1; base += kTile) {
-    const int cnt = (int)min((int64_t)kTile, s1 - base);
+  for (int64_t base = s0; base < s1; base += TILE) {
+    const int cnt = (int)min((int64_t)TILE, s1 - base);
     const int cntp = (cnt + kSub * G - 1) / (kSub * G) * (kSub * G);
-    __syncthreads();
-    if (tid < cnt) {
-      const int64_t s = base + tid;
-      s_x[tid] = (R)d.src[3 * s];
-      s_y[tid] = (R)d.src[3 * s + 1];
-      s_z[tid] = (R)d.src[3 * s + 2];
-      source_terms<PREC, kExpKind>(d, s, &s_a[tid], &s_w[tid]);
-    } else if (tid < cntp) {
-      s_x[tid] = s_y[tid] = s_z[tid] = (R)0;
-      s_a[tid] = (R)-INFINITY;
-      s_w[tid] = (R)0;
+    group_sync<NTHR>();
+    for (int i = tid; i < cntp; i += NTHR) {
+      if (i < cnt) {
+        const int64_t s = base + i;
+        SrcPt<R> sp;
+        sp.x = (R)(kScaled ? d.src[3 * s] * csc : d.src[3 * s]);
+        sp.y = (R)(kScaled ? d.src[3 * s + 1] * csc : d.src[3 * s + 1]);
+        sp.z = (R)(kScaled ? d.src[3 * s + 2] * csc : d.src[3 * s + 2]);
+        R wsrc;
+        source_terms<PREC, kExpKind>(d, s, &sp.a, &wsrc);
+        s_p[i] = sp;
+        s_w[i] = wsrc;
+      } else {
+        s_p[i] = SrcPt<R>{(R)0, (R)0, (R)0, (R)-INFINITY};
+        s_w[i] = (R)0;
+      }
     }
-    __syncthreads();
+    group_sync<NTHR>();
 
     if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
       for (int j = g; j < cnt; j += G) {
-        const R sx = s_x[j], sy = s_y[j], sz = s_z[j], sa = s_a[j];
+        const SrcPt<R> sp = s_p[j];
+        const R sx = sp.x, sy = sp.y, sz = sp.z, sa = sp.a;
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
           const R C = pair_cost<COST>(tx[r], ty[r], tz[r], sx, sy, sz, pexp);
@@ -534,15 +584,22 @@ __device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, cons
 #pragma unroll
         for (int u = 0; u < kSub; ++u) {
           const int j = j0 + u * G + g;
-          const R sx = s_x[j], sy = s_y[j], sz = s_z[j];
-          const R sa = s_a[j];
+          const SrcPt<R> sp = s_p[j];
+          const R sx = sp.x, sy = sp.y, sz = sp.z;
+          const R sa = sp.a;
           wv[u] = s_w[j];
           px[u] = sx; py[u] = sy; pz[u] = sz;
 #pragma unroll
           for (int r = 0; r < RPT; ++r) {
-            const R C = pair_cost<COST>(tx[r], ty[r], tz[r], sx, sy, sz, pexp);
-            Cv[r][u] = C;
-            v[r][u] = fma(k, C, sa + (kLazy ? tqm[r] : tq[r]));
+            if constexpr (kScaled) {
+              const R dr = tx[r] - sx, dg = ty[r] - sy, db = tz[r] - sz;
+              Cv[r][u] = (R)0;
+              v[r][u] = fma(-dr, dr, fma(-dg, dg, fma(-db, db, sa + tqm[r])));
+            } else {
+              const R C = pair_cost<COST>(tx[r], ty[r], tz[r], sx, sy, sz, pexp);
+              Cv[r][u] = C;
+              v[r][u] = fma(k, C, sa + (kLazy ? tqm[r] : tq[r]));
+            }
           }
         }
 #pragma unroll
@@ -555,11 +612,99 @@ __device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, cons
   }
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    if constexpr (kLazy) st[r].S *= exp2(-dl[r]);
+    if constexpr (kLazy) st[r].S *= exp2_small(-dl[r]);
     combine_lanes<PREC, KIND, G, R>(st[r]);
-    if (g != 0) tidx[r] = ntp;  // one owner lane per target
+    if (g != 0) tidx[r] = none;  // one owner lane per target
   }
+}
+
+template <int PREC, int KIND, int COST, int RPT>
+__device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, const int64_t tb,
+                                          const int split, const int splits,
+                                          const int64_t split_len, const int64_t ntp) {
+  using R = typename std::conditional<PREC == 0, double, float>::type;
+  constexpr int TPB = kThreads / kLanesPerTarget;
+  __shared__ SrcPt<R> s_p[kTile];
+  __shared__ R s_w[kTile];
+  const int64_t s0 = (int64_t)split * split_len;
+  const int64_t s1 = min(d.ns, s0 + split_len);
+  int64_t tidx[RPT];
+  TState<R> st[RPT];
+  pair_accum<PREC, KIND, COST, RPT, kThreads, kTile, R>(d, s_p, s_w, threadIdx.x, tb * TPB * RPT,
+                                                         s0, s1, ntp, tidx, st);
   finish_pass<PREC, KIND, RPT, R>(d, w, splits, split, ntp, tidx, st);
+}
+
+// Workgroup-wide splits: a workgroup of kWideWaves waves owns kWideTargets
+// targets (one wave's worth at kLanesPerTarget lanes per target); wave q takes
+// source split q, so the kWideWaves splits of a target are merged in LDS by
+// the same workgroup, with no partials in HBM and no second launch.  Each wave
+// stages its own split's sources.  The per-split states and their merge are
+// the split path's (pair_body with the same split_len, then merge_body), so
+// both give identical results.
+template <int PREC, int KIND, int COST>
+__global__ __launch_bounds__(kWideThreads) void pair_pass_wide(const PassDesc din,
+                                                               const int64_t split_len) {
+  using R = typename std::conditional<PREC == 0, double, float>::type;
+  constexpr int NF = kind_fields<KIND>();
+  constexpr int TILE = PREC == 1 ? kTile : kTile / 2;  // 80 KiB of LDS records either way
+  __shared__ SrcPt<R> s_p[kWideWaves][TILE];
+  __shared__ R s_w[kWideWaves][TILE];
+  __shared__ double s_part[NF][kWideWaves][kWideTargets];
+  const PassDesc d = resolve_desc(din);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int64_t t0 = (int64_t)blockIdx.x * kWideTargets;
+  const int64_t s0 = (int64_t)wave * split_len;
+  const int64_t s1 = min(d.ns, s0 + split_len);
+  const int nsplit = (int)((d.ns + split_len - 1) / split_len);
+  const int64_t none = -1;
+  int64_t tidx[1];
+  TState<R> st[1];
+  pair_accum<PREC, KIND, COST, 1, 64, TILE, R>(d, s_p[wave], s_w[wave], lane, t0, s0, s1, none,
+                                                tidx, st);
+  if (tidx[0] != none) {
+    const int tl = (int)(tidx[0] - t0);
+    double f[5];
+    if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) f[0] = (double)st[0].acc;
+    else if constexpr (KIND == KIND_SUM) { f[0] = st[0].S; f[1] = st[0].Sc; }
+    else { f[0] = (double)st[0].m; f[1] = st[0].S; f[2] = st[0].Sy0; f[3] = st[0].Sy1; f[4] = st[0].Sy2; }
+#pragma unroll
+    for (int q = 0; q < NF; ++q) s_part[q][wave][tl] = f[q];
+  }
+  __syncthreads();
+  // merge_body over the LDS partials: kMergeLanes lanes per target, lane
+  // `sub` holding split `sub`
+  if (threadIdx.x >= kWideTargets * kMergeLanes) return;
+  const int sub = threadIdx.x % kMergeLanes, tl = threadIdx.x / kMergeLanes;
+  const int64_t t = t0 + tl;
+  const bool ok = t < d.nt;
+  TState<R> acc;
+  acc.m = empty_max<PREC, R>();
+  acc.S = acc.Sc = acc.Sy0 = acc.Sy1 = acc.Sy2 = 0.0;
+  acc.acc = KIND == KIND_MIN ? (R)INFINITY : (R)-INFINITY;
+  if (ok && sub < nsplit) {
+    TState<R> p = acc;
+    if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
+      p.acc = (R)s_part[0][sub][tl];
+    } else if constexpr (KIND == KIND_SUM) {
+      p.S = s_part[0][sub][tl];
+      p.Sc = s_part[1][sub][tl];
+    } else {
+      p.m = (R)s_part[0][sub][tl];
+      p.S = s_part[1][sub][tl];
+      if (NF == 5) {
+        p.Sy0 = s_part[2][sub][tl];
+        p.Sy1 = s_part[3][sub][tl];
+        p.Sy2 = s_part[4][sub][tl];
+      }
+    }
+    combine_state<PREC, KIND, R>(acc, p);
+  }
+  combine_lanes<PREC, KIND, kMergeLanes, R>(acc);
+  if (!ok || sub != 0) return;
+  const double q = target_offset<PREC, KIND>(d, t);
+  acc.qd = PREC == 1 ? (double)(float)(q * kLog2e) : q;
+  epilogue_one<PREC, KIND, R>(d, t, acc);
 }
 
 template <int PREC, int KIND, int COST, int RPT>
@@ -1075,6 +1220,13 @@ hipError_t launch_mfma_ks(const PassDesc& d, const LaunchCfg& c, const Work& w, 
 
 template <int PREC, int KIND, int COST>
 hipError_t launch_rpt(const PassDesc& d, const LaunchCfg& c, const Work& w, hipStream_t s) {
+  if constexpr (KIND != KIND_BARY) {  // (five FP64 fields do not fit the wide layout's registers)
+    if (c.wide) {
+      hipLaunchKernelGGL((pair_pass_wide<PREC, KIND, COST>), dim3((unsigned)c.tblocks),
+                         dim3(kWideThreads), 0, s, d, c.split_len);
+      return hipGetLastError();
+    }
+  }
   dim3 grid((unsigned)c.tblocks, (unsigned)c.splits);
   switch (c.rpt) {
     case 1: hipLaunchKernelGGL((pair_pass<PREC, KIND, COST, 1>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;

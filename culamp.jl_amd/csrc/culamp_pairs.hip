@@ -57,16 +57,18 @@ int pass_fields(int kind) {
 
 // Launch shape of the point kernel: kLanesPerTarget lanes share a target, so
 // a workgroup owns (256 / kLanesPerTarget) * rpt targets.  Take the largest
-// rpt that still gives ~8 workgroups per CU once the sources are split into
-// chunks of >= 256 points (one LDS tile); fewer, longer splits keep the
-// merge traffic small.
+// rpt (at most 2) that still gives ~8 workgroups per CU once the sources are
+// split into chunks of >= 256 points (one LDS tile); fewer, longer splits
+// keep the merge traffic small.
 LaunchCfg choose_launch(int64_t nt, int64_t ns, int precision, int kind,
-                        int num_cu, int force_rpt, int force_splits) {
+                        int num_cu, int force_rpt, int force_splits, int layout) {
   LaunchCfg c{};
   const int64_t want = (int64_t)8 * (num_cu > 0 ? num_cu : 256);
   const int64_t tpb = kThreads / kLanesPerTarget;
   const bool fp64_only = kind == KIND_BARY || kind == KIND_MAX || kind == KIND_MIN;
-  const int max_rpt = (precision == 1 && !fp64_only) ? 4 : 2;
+  // (4 targets per thread measured slower than 2 in mixed precision at
+  // 8192-16384 targets: the extra registers cost more than the saved LDS reads)
+  const int max_rpt = 2;
   int rpt = 1;
   int64_t splits = 1;
   const int64_t cap = (ns + kTile - 1) / kTile;
@@ -79,7 +81,8 @@ LaunchCfg choose_launch(int64_t nt, int64_t ns, int precision, int kind,
     splits = sp;
     if (tb * sp >= want) break;
   }
-  if (force_rpt == 1 || force_rpt == 2 || (force_rpt == 4 && max_rpt == 4)) rpt = force_rpt;
+  const bool allow4 = precision == 1 && !fp64_only;  // (forced shapes only)
+  if (force_rpt == 1 || force_rpt == 2 || (force_rpt == 4 && allow4)) rpt = force_rpt;
   if (force_splits > 0) splits = force_splits;
   if (splits > 65535) splits = 65535;
   int64_t len = (ns + splits - 1) / splits;
@@ -91,6 +94,27 @@ LaunchCfg choose_launch(int64_t nt, int64_t ns, int precision, int kind,
   c.split_len = len;
   c.tblocks = (nt + tpb * rpt - 1) / (tpb * rpt);
   if (c.tblocks < 1) c.tblocks = 1;
+  // Workgroup-wide splits when one target per lane group leaves many splits
+  // to merge and the targets alone fill every CU (kWideTargets per
+  // workgroup): the split merge then needs no second launch.  A forced
+  // launch shape keeps the split layout unless the wide one is asked for.
+  const bool forced = force_rpt != 0 || force_splits != 0;
+  const bool auto_wide = !forced && layout == LAYOUT_AUTO && precision == 1 && !fp64_only &&
+                         c.rpt == 1 && c.splits >= 8 &&
+                         (nt + kWideTargets - 1) / kWideTargets >= (num_cu > 0 ? num_cu : 256);
+  if ((layout == LAYOUT_WIDE || auto_wide) && kind != KIND_BARY) {
+    int64_t want_splits = force_splits > 0 ? force_splits : kWideWaves;
+    if (want_splits > kWideWaves) want_splits = kWideWaves;
+    int64_t wl = (ns + want_splits - 1) / want_splits;
+    wl = (wl + kSub - 1) / kSub * kSub;
+    if (wl < kSub) wl = kSub;
+    c.wide = 1;
+    c.rpt = 1;
+    c.split_len = wl;
+    c.splits = ns > 0 ? (int)((ns + wl - 1) / wl) : 1;
+    c.tblocks = (nt + kWideTargets - 1) / kWideTargets;
+    if (c.tblocks < 1) c.tblocks = 1;
+  }
   return c;
 }
 

@@ -97,6 +97,7 @@ SIGNATURES = {
     "culamp_ctx_destroy": (_I, [_P]),
     "culamp_ctx_synchronize": (_I, [_P]),
     "culamp_ctx_set_launch": (_I, [_P, _I, _I]),
+    "culamp_ctx_set_layout": (_I, [_P, _I]),
     "culamp_malloc": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
     "culamp_free": (_I, [_P, _P]),
     "culamp_memcpy_htod": (_I, [_P, _P, _P, C.c_size_t]),
@@ -104,6 +105,10 @@ SIGNATURES = {
     "culamp_memcpy_dtod": (_I, [_P, _P, _P, C.c_size_t]),
     "culamp_timer_start": (_I, [_P]),
     "culamp_timer_stop": (_I, [_P, C.POINTER(_D)]),
+    "culamp_capture_begin": (_I, [_P]),
+    "culamp_capture_end": (_I, [_P, C.POINTER(_P)]),
+    "culamp_graph_launch": (_I, [_P, _P]),
+    "culamp_graph_destroy": (_I, [_P]),
     "culamp_max_cost": (_I, [_P, _P, _I64, _P, _I64, _D, _P]),
     "culamp_lse_rows": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _D, _D, _D, _I, _P]),
     "culamp_col_marginal_cost": (_I, [_P, _P, _I64, _P, _I64, _P, _P, _P, _D, _D, _D, _D, _I, _P, _P]),
@@ -237,6 +242,24 @@ class Context:
     def set_launch(self, rows_per_thread: int = 0, splits: int = 0) -> None:
         check(self.lib.culamp_ctx_set_launch(self.handle, rows_per_thread, splits))
 
+    def set_layout(self, layout: str = "auto") -> None:
+        """Source-split layout of point passes: "auto", "split" (over
+        workgroups, merged by a second launch) or "wide" (inside one
+        workgroup, merged in LDS)."""
+        codes = {"auto": 0, "split": 1, "wide": 2}
+        check(self.lib.culamp_ctx_set_layout(self.handle, codes[layout]))
+
+    def capture(self, fn) -> "Graph":
+        """Records the kernel-level calls made by fn() into a replayable graph."""
+        check(self.lib.culamp_capture_begin(self.handle))
+        try:
+            fn()
+        finally:
+            g = C.c_void_p()
+            rc = self.lib.culamp_capture_end(self.handle, C.byref(g))
+        check(rc)
+        return Graph(self, g)
+
     def timer_start(self) -> None:
         check(self.lib.culamp_timer_start(self.handle))
 
@@ -244,6 +267,28 @@ class Context:
         ms = C.c_double(0.0)
         check(self.lib.culamp_timer_stop(self.handle, C.byref(ms)))
         return ms.value
+
+
+class Graph:
+    """A captured sequence of kernel-level calls (Context.capture)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx, self.handle = ctx, handle
+        ctx.adopt(self)
+
+    def launch(self) -> None:
+        check(self.ctx.lib.culamp_graph_launch(self.ctx.handle, self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx.lib.culamp_graph_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceArray:

@@ -68,6 +68,7 @@ enum {
 typedef struct culamp_ctx culamp_ctx;
 typedef struct culamp_problem culamp_problem;
 typedef struct culamp_batch culamp_batch;
+typedef struct culamp_graph culamp_graph;
 
 /* Mirrors EOTArgs (src/Utilities.jl:18-31) field for field, same defaults via
  * culamp_args_default.  Two fields are new and default to the reference's
@@ -118,6 +119,13 @@ int culamp_ctx_synchronize(culamp_ctx *ctx);
  * tuner): targets per thread in {1, 2, 4} and number of source splits; 0
  * selects automatically. */
 int culamp_ctx_set_launch(culamp_ctx *ctx, int rows_per_thread, int splits);
+/* Layout of the source splits of point-cloud passes (no reference
+ * counterpart; tests and tuning): 0 automatic, 1 splits over workgroups
+ * merged by a second launch, 2 up to 16 splits inside one workgroup of 16
+ * waves, merged in LDS.  With the same split length both give identical
+ * results.  With layout 2 a nonzero `splits` of culamp_ctx_set_launch caps
+ * the in-workgroup split count. */
+int culamp_ctx_set_layout(culamp_ctx *ctx, int layout);
 /* Device memory helpers (stream-ordered on the context's stream). */
 int culamp_malloc(culamp_ctx *ctx, size_t bytes, void **dptr);
 int culamp_free(culamp_ctx *ctx, void *dptr);
@@ -128,6 +136,16 @@ int culamp_memcpy_dtod(culamp_ctx *ctx, void *dst, const void *src, size_t bytes
  * elapsed milliseconds between the two events (waits for the stop event). */
 int culamp_timer_start(culamp_ctx *ctx);
 int culamp_timer_stop(culamp_ctx *ctx, double *ms);
+
+/* Stream capture (no reference counterpart): kernel-level calls issued on the
+ * context between begin and end are recorded into a HIP graph instead of
+ * running, and run each time the graph is launched (stream-ordered).  Issue
+ * each call once before capturing it, so that its work buffers exist (nothing
+ * is allocated while capturing); solver-level calls cannot be captured. */
+int culamp_capture_begin(culamp_ctx *ctx);
+int culamp_capture_end(culamp_ctx *ctx, culamp_graph **graph);
+int culamp_graph_launch(culamp_ctx *ctx, culamp_graph *graph);
+int culamp_graph_destroy(culamp_graph *graph);
 
 /* ---- kernel level (device pointers) -------------------------------------- */
 /* max_logsumexp_spp_ct! (src/solvers/MirrorProxKernel.jl:280-326):

@@ -171,6 +171,88 @@ def test_deterministic_repeat(gpu_ctx, oracle):
             assert np.array_equal(a, o.get())
 
 
+@pytest.mark.parametrize("n,m", [(4096, 4096), (4100, 333), (700, 5000), (16, 9), (1, 7)])
+def test_wide_layout_matches_split_layout(gpu_ctx, oracle, n, m):
+    """Source splits merged inside one workgroup (wide layout) and over
+    workgroups with a merge launch (split layout) give identical bits for the
+    same split length, for every kind the wide layout runs, both precisions
+    and every cost; the wide FP64 row LSE also matches the oracle."""
+    from culamp_amd import FP64, MIXED
+    from culamp_amd import kernels as K
+    say("wide vs split n", n, "m", m)
+    x, y, th, r = _inputs(3 * n + m, n, m)
+    dx, dy, dth, dr = dev(gpu_ctx, x), dev(gpu_ctx, y), dev(gpu_ctx, th), dev(gpu_ctx, r)
+    lz = dev(gpu_ctx, np.linspace(-1.0, 2.0, n))
+    o, res, cost = out(gpu_ctx, n), out(gpu_ctx, m), out(gpu_ctx, m)
+
+    def run_all(p):
+        got = []
+        for prec in (FP64, MIXED):
+            K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, dy, dth, 0.02, 1.0, 3.0, p, prec)
+            got.append(o.get())
+            K.residual_spp_c(gpu_ctx, res, cost, dx, dy, dr, dth, lz, 0.02, 1.0, 3.0, p, prec)
+            got += [res.get(), cost.get()]
+        K.max_logsumexp_spp_ct(gpu_ctx, o, dx, dy, p)
+        got.append(o.get())
+        K.warp_min_reduce(gpu_ctx, o, dx, dy, dth, 3.0, p)
+        got.append(o.get())
+        return got
+
+    try:
+        for p in (1.0, 2.0, math.inf, 3.0):
+            for splits in (16, 5):
+                gpu_ctx.set_layout("split")
+                gpu_ctx.set_launch(1, splits)
+                want = run_all(p)
+                gpu_ctx.set_layout("wide")
+                gpu_ctx.set_launch(0, splits)
+                for rep in range(2):
+                    got = run_all(p)
+                    for k, (w, g) in enumerate(zip(want, got)):
+                        assert np.array_equal(w, g), (p, splits, rep, k)
+            rows = np.arange(min(n, 24))
+            ref = oracle.lse_rows(x[rows], y, th, 0.02, 1.0, 3.0, p)
+            K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, dy, dth, 0.02, 1.0, 3.0, p, FP64)
+            assert rel(o.get()[rows], ref) < FP64_REL, p
+    finally:
+        gpu_ctx.set_launch(0, 0)
+        gpu_ctx.set_layout("auto")
+
+
+def test_captured_graph_replays_match_eager(gpu_ctx, oracle):
+    """Kernel-level calls captured into a graph: every replay equals the eager
+    result, with new inputs uploaded and eager work run between the replays."""
+    from culamp_amd import FP64, MIXED
+    from culamp_amd import kernels as K
+    x, y, th, r = _inputs(13, 2000, 3000)
+    dx, dy, dth, dr = dev(gpu_ctx, x), dev(gpu_ctx, y), dev(gpu_ctx, th), dev(gpu_ctx, r)
+    lz, res, cost = out(gpu_ctx, 2000), out(gpu_ctx, 3000), out(gpu_ctx, 3000)
+
+    def step(prec):
+        K.warp_logsumexp_spp_ct_fused(gpu_ctx, lz, dx, dy, dth, 0.02, 1.0, 3.0, 2.0, prec)
+        K.residual_spp_c(gpu_ctx, res, cost, dx, dy, dr, dth, lz, 0.02, 1.0, 3.0, 2.0, prec)
+
+    rng = np.random.default_rng(4)
+    for prec in (FP64, MIXED):
+        step(prec)  # work buffers exist before capture
+        graph = gpu_ctx.capture(lambda: step(prec))
+        for rep in range(4):
+            th_r = th + rng.normal(0, 0.3, th.size)
+            dth.upload(th_r)
+            step(prec)
+            want = (lz.get(), res.get(), cost.get())
+            lz.upload(np.zeros(2000))
+            res.upload(np.zeros(3000))
+            graph.launch()
+            got = (lz.get(), res.get(), cost.get())
+            for w, g in zip(want, got):
+                assert np.array_equal(w, g), (prec, rep)
+            if prec == FP64:
+                ref = oracle.lse_rows(x[:32], y, th_r, 0.02, 1.0, 3.0, 2.0)
+                assert rel(got[0][:32], ref) < FP64_REL
+        graph.close()
+
+
 @pytest.mark.parametrize("n,m", [(1, 1), (5, 3), (700, 1100), (4096, 4096)])
 @pytest.mark.parametrize("p", [1.0, 2.0, math.inf, 3.0])
 def test_sinkhorn_kernels_vs_oracle(gpu_ctx, oracle, n, m, p):
