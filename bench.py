@@ -19,8 +19,9 @@ Printed as one JSON line (rank 0):
 Extra fields: solves/s and iterations/s for mixed and for the plain FP64
 single-instance path (host-checked stop every `frequency` iterations), the
 mixed-vs-FP64 error of the transport cost, `batched` (256 cell-similarity
-instances in one batch vs the plain FP64 path one instance at a time) and
-`divergence` (Sinkhorn divergence with gradients, 65536 points).
+instances in one batch vs the plain FP64 path one instance at a time),
+`divergence` (Sinkhorn divergence with gradients, 65536 points) and `cindex`
+(cell-similarity C-index over all cell pairs, batched vs plain).
 """
 from __future__ import annotations
 
@@ -113,6 +114,52 @@ def bench_batched(ctx, a, log) -> dict:
         pb.close()
     b.close()
     return out
+
+
+def bench_cindex(ctx, a, log) -> dict:
+    """Cell-similarity C-index (the reference's multi-instance workload,
+    src/CellSimilarity.jl:164-255): every pair of `cindex_cells` cells of
+    synthetic clustered data with 96 features, cosine cost between features,
+    LAMP distances to L1 infeasibility 1e-6 (eta_p = 5e-2); all pairs in
+    batches on one shared device copy of the features (bf16 MFMA), against
+    the plain FP64 single-instance path pair by pair (time-capped sample)."""
+    import numpy as np
+    import culamp_amd
+    from culamp_amd import cindex
+    cd = cindex.synthetic_cell_data(96, a.cindex_cells, 4, seed=3)
+    kern = cindex.CellCostKernel.build(cd.data, "cosine")
+    kw = dict(eta_p=5e-2, epsilon=6e-6, B=1.1, itermax=20000, verbose=False, stop="infeas")
+    npairs = a.cindex_cells * (a.cindex_cells - 1) // 2
+    args = culamp_amd.EOTArgs(precision=a.batch_precision, **kw)
+    cindex.pair_distances(kern, cindex._pair_order(a.cindex_cells, 64), args, a.frequency, "lamp",
+                          64, ctx)  # warm-up
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    out = cindex.streaming_c_index(cd, kern, args, npairs, frequency=a.frequency, solver="lamp",
+                                   batch_size=a.cindex_batch, ctx=ctx)
+    dt = time.perf_counter() - t0
+    pairs = cindex._pair_order(a.cindex_cells, npairs)
+    f64 = culamp_amd.EOTArgs(precision="fp64", **kw)
+    sums = cd.data.sum(axis=0)
+    data = np.asfortranarray(cd.data)
+    done, t1 = 0, time.perf_counter()
+    for i, j in pairs:
+        cindex.extragradient_cell_distance(data, cindex.cell_marginal(cd.data, i, sums),
+                                           cindex.cell_marginal(cd.data, j, sums), kern.scale,
+                                           "cosine", f64, a.frequency, ctx)
+        done += 1
+        if time.perf_counter() - t1 > a.plain_seconds / 4:
+            break
+    plain_rate = done / (time.perf_counter() - t1)
+    log(f"c-index: {npairs} pairs in {dt:.2f}s batched ({npairs / dt:.0f} pairs/s), "
+        f"plain fp64 {plain_rate:.1f} pairs/s, c_index {out['c_index']:.4f}")
+    return {"workload": f"C-index over all {npairs} pairs of {a.cindex_cells} synthetic cells "
+                        "(96 features, 4 clusters), cosine/2 cost between features, LAMP to L1 "
+                        "infeasibility 1e-6, eta_p=5e-2",
+            "precision": a.batch_precision, "pairs": npairs, "seconds": dt,
+            "pairs_per_s": npairs / dt, "c_index": out["c_index"],
+            "plain_fp64_pairs_per_s": plain_rate, "plain_fp64_pairs_timed": done,
+            "speedup_vs_plain_fp64": (npairs / dt) / plain_rate}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_lse_mixed_grid64.json")
@@ -249,6 +296,9 @@ def main() -> None:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--plain-seconds", type=float, default=60.0,
                     help="time budget of the sequential FP64 single-instance comparison")
+    ap.add_argument("--cindex-cells", type=int, default=64)
+    ap.add_argument("--cindex-batch", type=int, default=2016)
+    ap.add_argument("--no-cindex", action="store_true")
     ap.add_argument("--diff-n", type=int, default=65536)
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
@@ -344,6 +394,7 @@ def main() -> None:
 
     batched = None if a.no_batch else bench_batched(ctx, a, log)
     divergence = None if a.no_diff else bench_divergence(ctx, a, log)
+    cidx = None if a.no_cindex else bench_cindex(ctx, a, log)
 
     ms_per_step = dt / a.steps * 1e3
     rec = {
@@ -364,6 +415,8 @@ def main() -> None:
         rec["batched"] = batched
     if divergence is not None:
         rec["divergence"] = divergence
+    if cidx is not None:
+        rec["cindex"] = cidx
     print(json.dumps(rec), flush=True)
 
 

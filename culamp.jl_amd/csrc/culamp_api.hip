@@ -73,6 +73,33 @@ int ensure_work(culamp_ctx* ctx, size_t partial_doubles) {
   return CULAMP_OK;
 }
 
+int alias_problem(culamp_ctx* ctx, const culamp_problem* owner, const double* r,
+                  const double* c, culamp_problem** out) {
+  *out = nullptr;
+  culamp_problem* pb = new culamp_problem(*owner);
+  pb->geom_owner = owner->geom_owner ? owner->geom_owner : owner;
+  pb->r = pb->c = pb->logr = pb->logc = nullptr;
+  pb->hr.assign(r, r + pb->n);
+  pb->hc.assign(c, c + pb->m);
+  hipError_t e = hipSuccess;
+  for (double** p : {&pb->r, &pb->logr})
+    if (e == hipSuccess) e = hipMalloc(p, sizeof(double) * pb->n);
+  for (double** p : {&pb->c, &pb->logc})
+    if (e == hipSuccess) e = hipMalloc(p, sizeof(double) * pb->m);
+  if (e == hipSuccess) e = hipMemcpyAsync(pb->r, r, sizeof(double) * pb->n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(pb->c, c, sizeof(double) * pb->m, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_log(pb->logr, pb->r, pb->n, ctx->stream);
+  if (e == hipSuccess) e = launch_log(pb->logc, pb->c, pb->m, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->problems.insert(pb);
+  if (e != hipSuccess) {
+    culamp_problem_destroy(pb);
+    return fail(CULAMP_ERR_HIP, "problem upload failed: %s", hipGetErrorString(e));
+  }
+  *out = pb;
+  return CULAMP_OK;
+}
+
 int run_pass(culamp_ctx* ctx, const PassDesc& d, int precision, int kind) {
   if (d.nt <= 0) return CULAMP_OK;
   if (precision < CULAMP_FP64 || precision > CULAMP_BF16X3)
@@ -746,6 +773,12 @@ int culamp_problem_destroy(culamp_problem* pb) {
   pb->ctx->problems.erase(pb);
   (void)hipSetDevice(pb->ctx->device);
   (void)hipStreamSynchronize(pb->ctx->stream);
+  if (pb->geom_owner) {
+    for (double* p : {pb->r, pb->c, pb->logr, pb->logc})
+      if (p) (void)hipFree(p);
+    delete pb;
+    return CULAMP_OK;
+  }
   for (double* p : {pb->x, pb->y, pb->r, pb->c, pb->logr, pb->logc, pb->xf64, pb->xrc})
     if (p) (void)hipFree(p);
   if (pb->xf32) (void)hipFree(pb->xf32);

@@ -27,6 +27,8 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "culamp_host.h"
@@ -958,11 +960,22 @@ int culamp_batch_create(culamp_ctx* ctx, const culamp_instance* inst, int32_t co
   b->tune.graph = 1;
   ctx->batches.insert(b);
   int rc = CULAMP_OK;
+  // instances given the same x / y arrays (same host pointers and sizes)
+  // share one device copy of their points or feature rows
+  std::map<std::tuple<const double*, int64_t, const double*, int64_t>, culamp_problem*> geom;
   for (int32_t k = 0; k < count && rc == CULAMP_OK; ++k) {
     const culamp_instance& I = inst[k];
     culamp_problem* pb = nullptr;
-    if (d > 0) rc = culamp_feature_problem_create(ctx, I.x, I.n, I.y, I.m, d, metric, scale, I.r, I.c, &pb);
-    else rc = culamp_problem_create(ctx, I.x, I.n, I.y, I.m, I.r, I.c, &pb);
+    const double* ykey = (d > 0 && I.y == nullptr) ? I.x : I.y;
+    const auto key = std::make_tuple(I.x, I.n, ykey, (d > 0 && I.y == nullptr) ? I.n : I.m);
+    const auto it = geom.find(key);
+    if (it != geom.end() && I.r != nullptr && I.c != nullptr) {
+      rc = alias_problem(ctx, it->second, I.r, I.c, &pb);
+    } else {
+      if (d > 0) rc = culamp_feature_problem_create(ctx, I.x, I.n, I.y, I.m, d, metric, scale, I.r, I.c, &pb);
+      else rc = culamp_problem_create(ctx, I.x, I.n, I.y, I.m, I.r, I.c, &pb);
+      if (rc == CULAMP_OK) geom.emplace(key, pb);
+    }
     if (rc != CULAMP_OK) {
       const std::string msg = g_err;
       culamp_batch_destroy(b);

@@ -43,6 +43,9 @@ struct culamp_problem {
   double *xf64 = nullptr, *yf64 = nullptr, *xrc = nullptr, *yrc = nullptr;
   float *xf32 = nullptr, *yf32 = nullptr;
   bool self = false;  // y is x
+  // shares the point / feature buffers of another problem (same inputs, own
+  // marginals); only r, c, logr, logc belong to this problem
+  const culamp_problem* geom_owner = nullptr;
   // MFMA path (cosine / correlation): unit rows in bf16, width 16 * zks
   uint16_t *xz_hi = nullptr, *xz_lo = nullptr, *yz_hi = nullptr, *yz_lo = nullptr;
   int zks = 0;  // 0: not eligible
@@ -71,6 +74,9 @@ double now_s();
 std::string fmt(const char* f, double v);
 void emit(culamp_log_fn fn, void* user, const std::string& s);
 int ensure_work(culamp_ctx* ctx, size_t partial_doubles);
+// A problem on `owner`'s geometry with its own marginals (n, m as owner's).
+int alias_problem(culamp_ctx* ctx, const culamp_problem* owner, const double* r,
+                  const double* c, culamp_problem** out);
 int run_pass(culamp_ctx* ctx, const PassDesc& d, int precision, int kind);
 PassDesc blank();
 void feat(PassDesc& d, const culamp_problem* pb, bool rows);

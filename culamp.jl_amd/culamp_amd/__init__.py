@@ -20,6 +20,9 @@ from .diff import divergence_opts, sinkhorn_divergence
 from . import diff
 from .cellsim import (FeatureProblem, annealed_sinkhorn_cellsim, cost_scale,
                       extragradient_cellsim, normalize_cell_cost_metric, sinkhorn_cellsim)
+from . import cindex
+from .cindex import (CellCostKernel, OTScOmicsCellData, compute_otscomics_c_index,
+                     streaming_c_index)
 
 __all__ = [
     "BF16", "BF16X3", "FP64", "MIXED", "CulampError", "Context", "DeviceArray", "device_count", "load",
@@ -28,5 +31,6 @@ __all__ = [
     "sinkhorn_color_transfer", "sinkhorn_euclidean", "Problem", "kernels", "FeatureProblem",
     "annealed_sinkhorn_cellsim", "cost_scale", "extragradient_cellsim",
     "normalize_cell_cost_metric", "sinkhorn_cellsim", "Batch", "divergence_opts",
-    "sinkhorn_divergence", "diff",
+    "sinkhorn_divergence", "diff", "cindex", "CellCostKernel", "OTScOmicsCellData",
+    "compute_otscomics_c_index", "streaming_c_index",
 ]

@@ -33,10 +33,23 @@ class Batch:
         self.metric = normalize_cell_cost_metric(metric) if self.feature else None
         keep, descs = [], []
         d = 0
+        # each distinct input array is converted once, so instances that pass
+        # the same array share one device copy of it (culamp_batch_create
+        # matches them by pointer)
+        conv = {}
+
+        def once(a, fn):
+            key = id(a)
+            if key not in conv:
+                conv[key] = (a, fn(a))
+            return conv[key][1]
+
+        fortran = lambda a: np.asfortranarray(a, dtype=np.float64)  # noqa: E731
+        contig = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
         for k, (x, y, r, c) in enumerate(instances):
             if self.feature:
-                xa = np.asfortranarray(x, dtype=np.float64)
-                ya = None if (y is None or y is x) else np.asfortranarray(y, dtype=np.float64)
+                xa = once(x, fortran)
+                ya = None if (y is None or y is x) else once(y, fortran)
                 if xa.ndim != 2 or (ya is not None and ya.shape[1] != xa.shape[1]):
                     raise ValueError(f"instance {k}: feature rows must be (points x d) with equal d")
                 if d and xa.shape[1] != d:
@@ -44,8 +57,8 @@ class Batch:
                 d = xa.shape[1]
                 n, m = xa.shape[0], (xa.shape[0] if ya is None else ya.shape[0])
             else:
-                xa = np.ascontiguousarray(x, dtype=np.float64)
-                ya = np.ascontiguousarray(y, dtype=np.float64)
+                xa = once(x, contig)
+                ya = once(y, contig)
                 if xa.ndim != 2 or xa.shape[1] != 3 or ya.ndim != 2 or ya.shape[1] != 3:
                     raise ValueError(f"instance {k}: point clouds must be (n, 3) arrays")
                 n, m = xa.shape[0], ya.shape[0]
