@@ -1023,7 +1023,10 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
                                           const int64_t split_len, const int64_t ntp) {
   constexpr int DP = 16 * KS;          // padded row width
   constexpr int LDW = DP + 8;          // LDS row stride (bf16), breaks bank alignment
-  constexpr int TILE = 32;
+  constexpr int SUB = 32;              // sources per MFMA tile
+  // sources staged per workgroup barrier: several MFMA tiles, so that the
+  // staging round trip and the two barriers are paid once per 2-4 tiles
+  constexpr int TILE = (NSPLIT == 3 || KS > 8) ? 64 : 128;
   __shared__ __bf16 Ah[TILE * LDW];
   __shared__ __bf16 Al[NSPLIT == 3 ? TILE * LDW : 1];
   __shared__ __align__(16) float s_a[TILE];
@@ -1075,97 +1078,123 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
 
   for (int64_t base = s0; base < s1; base += TILE) {
     const int cnt = (int)min((int64_t)TILE, s1 - base);
+    const int nsub = (cnt + SUB - 1) / SUB;
     __syncthreads();
-    // stage TILE source rows: TILE * DP bf16 = TILE * KS * 2 chunks of 16 B
-    for (int e = tid; e < TILE * KS * 2; e += kThreads) {
-      const int row = e / (KS * 2), c8 = (e % (KS * 2)) * 8;
-      bf16x8_t vh, vl;
-      if (row < cnt) {
-        vh = *(const bf16x8_t*)(Sh + (base + row) * DP + c8);
-        if constexpr (NSPLIT == 3) vl = *(const bf16x8_t*)(Sl + (base + row) * DP + c8);
-      } else {
+    // the source terms' loads go out first, with the row loads below
+    const bool term = tid < nsub * SUB;
+    const bool real = tid < cnt;
+    const double sa_raw = (real && d.srcA != nullptr) ? d.srcA[base + tid] : 0.0;
+    const double sb_raw = (real && d.srcB != nullptr) ? d.srcB[base + tid] : 1.0;
+    // stage the tile's source rows, zero padded to whole MFMA tiles:
+    // nsub * SUB * DP bf16 = nsub * SUB * KS * 2 chunks of 16 B; every load of
+    // a thread is issued before its first LDS write
+    {
+      constexpr int CHUNKS = TILE * KS * 2;
+      constexpr int PER = (CHUNKS + kThreads - 1) / kThreads;
+      const int live = nsub * SUB * KS * 2;
+      bf16x8_t vh[PER], vl[NSPLIT == 3 ? PER : 1];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) { vh[q] = (__bf16)0.0f; vl[q] = (__bf16)0.0f; }
+      for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * kThreads;
+        const int row = e / (KS * 2), c8 = (e % (KS * 2)) * 8;
+        if (e < live && row < cnt) {
+          vh[i] = *(const bf16x8_t*)(Sh + (base + row) * DP + c8);
+          if constexpr (NSPLIT == 3) vl[i] = *(const bf16x8_t*)(Sl + (base + row) * DP + c8);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            vh[i][q] = (__bf16)0.0f;
+            if constexpr (NSPLIT == 3) vl[i][q] = (__bf16)0.0f;
+          }
+        }
       }
-      *(bf16x8_t*)(Ah + row * LDW + c8) = vh;
-      if constexpr (NSPLIT == 3) *(bf16x8_t*)(Al + row * LDW + c8) = vl;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * kThreads;
+        if (e >= live) break;
+        const int row = e / (KS * 2), c8 = (e % (KS * 2)) * 8;
+        *(bf16x8_t*)(Ah + row * LDW + c8) = vh[i];
+        if constexpr (NSPLIT == 3) *(bf16x8_t*)(Al + row * LDW + c8) = vl[i];
+      }
     }
-    if (tid < TILE) {
+    if (term) {
       float a = -INFINITY, wv = 0.0f;
-      if (tid < cnt) {
+      if (real) {
         // fold the constant k / scale into the FP64 source term before the split
-        const double full = (d.srcA != nullptr ? d.alpha * d.srcA[base + tid] : 0.0) + d.k / d.fscale;
+        const double full = (d.srcA != nullptr ? d.alpha * sa_raw : 0.0) + d.k / d.fscale;
         const double a2 = full * kLog2e;
         const float hi = (float)a2;
         const double lo = a2 - (double)hi;
-        const double bw = d.srcB != nullptr ? d.srcB[base + tid] : 1.0;
         a = hi;
-        wv = (float)(isfinite(lo) ? bw * (1.0 + lo * kLn2) : bw);
+        wv = (float)(isfinite(lo) ? sb_raw * (1.0 + lo * kLn2) : sb_raw);
       }
       s_a[tid] = a;
       s_w[tid] = wv;
     }
     __syncthreads();
-    f32x16_t acc;
+    for (int sub = 0; sub < nsub; ++sub) {
+      const int r0 = sub * SUB;
+      f32x16_t acc;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+      for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const int k0 = ks * 16 + 8 * (lane >> 5);
-      const bf16x8_t ah = *(const bf16x8_t*)(Ah + jt * LDW + k0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[ks], acc, 0, 0, 0);
-      if constexpr (NSPLIT == 3) {
-        const bf16x8_t al = *(const bf16x8_t*)(Al + jt * LDW + k0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[ks], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[ks], acc, 0, 0, 0);
-      }
-    }
-    // the lane's 16 sources are 4 runs of 4 consecutive rows: i = 8 q + 4 (lane / 32) + e;
-    // padded sources carry a_s = -inf, w_s = 0
-    float sa[16], sw[16];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int i0 = 8 * q + 4 * (lane >> 5);
-      const float4 a4 = *(const float4*)(s_a + i0);
-      const float4 w4 = *(const float4*)(s_w + i0);
-      sa[4 * q] = a4.x; sa[4 * q + 1] = a4.y; sa[4 * q + 2] = a4.z; sa[4 * q + 3] = a4.w;
-      sw[4 * q] = w4.x; sw[4 * q + 1] = w4.y; sw[4 * q + 2] = w4.z; sw[4 * q + 3] = w4.w;
-    }
-    if constexpr (KIND == KIND_LSE) {
-      float v[16];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
-        v[r] = fmaf(kc, sim, sa[r] + tqm);
-      }
-      lse_lazy<16>(st, v, sw, tq, tqm, dl);
-    } else if (d.out1 != nullptr) {
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        float v[8], wv[8], Cv[8], dz[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int r = h * 8 + u;
-          const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
-          Cv[u] = (1.0f - sim) * inv_scale;
-          wv[u] = sw[r];
-          dz[u] = 0.0f;
-          v[u] = fmaf(kc, sim, sa[r] + tq);
+      for (int ks = 0; ks < KS; ++ks) {
+        const int k0 = ks * 16 + 8 * (lane >> 5);
+        const bf16x8_t ah = *(const bf16x8_t*)(Ah + (r0 + jt) * LDW + k0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[ks], acc, 0, 0, 0);
+        if constexpr (NSPLIT == 3) {
+          const bf16x8_t al = *(const bf16x8_t*)(Al + (r0 + jt) * LDW + k0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[ks], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[ks], acc, 0, 0, 0);
         }
-        accumulate<1, KIND, 8, float>(st, v, wv, Cv, dz, dz, dz);
       }
-    } else {  // no cost output: marginal only
-      float v[16], dz[16];
+      // the lane's 16 sources are 4 runs of 4 consecutive rows: i = 8 q + 4 (lane / 32) + e;
+      // padded sources carry a_s = -inf, w_s = 0
+      float sa[16], sw[16];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
-        dz[r] = 0.0f;
-        v[r] = fmaf(kc, sim, sa[r] + tq);
+      for (int q = 0; q < 4; ++q) {
+        const int i0 = r0 + 8 * q + 4 * (lane >> 5);
+        const float4 a4 = *(const float4*)(s_a + i0);
+        const float4 w4 = *(const float4*)(s_w + i0);
+        sa[4 * q] = a4.x; sa[4 * q + 1] = a4.y; sa[4 * q + 2] = a4.z; sa[4 * q + 3] = a4.w;
+        sw[4 * q] = w4.x; sw[4 * q + 1] = w4.y; sw[4 * q + 2] = w4.z; sw[4 * q + 3] = w4.w;
       }
-      accumulate<1, KIND, 16, float, false>(st, v, sw, dz, dz, dz, dz);
+      if constexpr (KIND == KIND_LSE) {
+        float v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
+          v[r] = fmaf(kc, sim, sa[r] + tqm);
+        }
+        lse_lazy<16>(st, v, sw, tq, tqm, dl);
+      } else if (d.out1 != nullptr) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          float v[8], wv[8], Cv[8], dz[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int r = h * 8 + u;
+            const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
+            Cv[u] = (1.0f - sim) * inv_scale;
+            wv[u] = sw[r];
+            dz[u] = 0.0f;
+            v[u] = fmaf(kc, sim, sa[r] + tq);
+          }
+          accumulate<1, KIND, 8, float>(st, v, wv, Cv, dz, dz, dz);
+        }
+      } else {  // no cost output: marginal only
+        float v[16], dz[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
+          dz[r] = 0.0f;
+          v[r] = fmaf(kc, sim, sa[r] + tq);
+        }
+        accumulate<1, KIND, 16, float, false>(st, v, sw, dz, dz, dz, dz);
+      }
     }
   }
-  if constexpr (KIND == KIND_LSE) st.S *= exp2(-dl);
+  if constexpr (KIND == KIND_LSE) st.S *= exp2_small(-dl);
   // merge the two half-waves holding the same target
   if constexpr (KIND == KIND_LSE) {
     const float mo = __shfl_xor(st.m, 32);
