@@ -242,16 +242,38 @@ __device__ __forceinline__ void finish_pass(const PassDesc& d, const Work& w,
 // Per-target exponent offset q_t: the target term of a SUM pass, or (mixed
 // LSE passes of the solver loops) minus the target's previous LSE so that
 // FP32 exponents stay near zero.  Returns q in natural units.
+// The per-target inputs of a point pass, loaded together (t must be a valid
+// target index) so that their round trips overlap.
+struct TgtLoad {
+  double x, y, z;     // coordinates (point passes)
+  double ta, prev, mg;  // tgtA[t], out0[t], marg[t] as target_offset needs them
+};
+
 template <int PREC, int KIND>
-__device__ __forceinline__ double target_offset(const PassDesc& d, int64_t t) {
+__device__ __forceinline__ TgtLoad load_offset_inputs(const PassDesc& d, const int64_t t) {
+  TgtLoad L;
+  L.x = L.y = L.z = 0.0;
+  L.ta = (KIND == KIND_SUM && d.tgtA != nullptr) ? d.tgtA[t] : 0.0;
+  const bool prev = PREC == 1 && KIND == KIND_LSE && d.prev_offset;
+  L.prev = prev ? d.out0[t] : 0.0;
+  L.mg = (prev && d.epi == EPI_SK_HALF) ? d.marg[t] : 1.0;
+  return L;
+}
+
+template <int PREC, int KIND>
+__device__ __forceinline__ double offset_from(const PassDesc& d, const TgtLoad& L) {
   double q = 0.0;
-  if (KIND == KIND_SUM && d.tgtA != nullptr) q = d.gamma * d.tgtA[t];
+  if (KIND == KIND_SUM && d.tgtA != nullptr) q = d.gamma * L.ta;
   if (PREC == 1 && KIND == KIND_LSE && d.prev_offset) {
-    const double prev = d.out0[t];
-    const double lse_prev = d.epi == EPI_SK_HALF ? log(d.marg[t]) - prev : prev;
+    const double lse_prev = d.epi == EPI_SK_HALF ? log(L.mg) - L.prev : L.prev;
     q = isfinite(lse_prev) ? -lse_prev : 0.0;
   }
   return q;
+}
+
+template <int PREC, int KIND>
+__device__ __forceinline__ double target_offset(const PassDesc& d, int64_t t) {
+  return offset_from<PREC, KIND>(d, load_offset_inputs<PREC, KIND>(d, t));
 }
 
 // Combine the split partials of each target, then apply the epilogue.
@@ -331,9 +353,10 @@ hipError_t launch_merge(const PassDesc& d, const LaunchCfg& c, const Work& w, in
 
 // Source exponent term and weight as staged in LDS (see the header comment).
 template <int PREC, bool EXPKIND, typename R>
-__device__ __forceinline__ void source_terms(const PassDesc& d, int64_t s, R* a_out, R* w_out) {
-  const double av = d.srcA != nullptr ? d.alpha * d.srcA[s] : 0.0;
-  const double bw = d.srcB != nullptr ? d.srcB[s] : 1.0;
+__device__ __forceinline__ void source_terms_v(const PassDesc& d, const double a_raw,
+                                               const double b_raw, R* a_out, R* w_out) {
+  const double av = d.srcA != nullptr ? d.alpha * a_raw : 0.0;
+  const double bw = d.srcB != nullptr ? b_raw : 1.0;
   if (PREC == 1 && EXPKIND) {
     const double a2 = av * kLog2e;
     const float hi = (float)a2;
@@ -344,6 +367,12 @@ __device__ __forceinline__ void source_terms(const PassDesc& d, int64_t s, R* a_
     *a_out = (R)av;
     *w_out = (R)bw;
   }
+}
+
+template <int PREC, bool EXPKIND, typename R>
+__device__ __forceinline__ void source_terms(const PassDesc& d, int64_t s, R* a_out, R* w_out) {
+  source_terms_v<PREC, EXPKIND, R>(d, d.srcA != nullptr ? d.srcA[s] : 0.0,
+                                   d.srcB != nullptr ? d.srcB[s] : 1.0, a_out, w_out);
 }
 
 // 2^x for the small arguments of lse_lazy's rescale (a difference of two
@@ -512,17 +541,44 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
   // differences, with no separate scaling of the cost
   constexpr bool kScaled = kLazy && COST == COST_SQL2;
   const double csc = kScaled ? sqrt(fmax(-(d.k * kLog2e), 0.0)) : 1.0;
+  // This thread's source of the next tile, held in registers from its load
+  // until it is written to LDS: the loads of a tile go out together and one
+  // tile ahead of the pair loop (every index is clamped into range, so no
+  // load waits on a branch).
+  static_assert(TILE == NTHR, "one staged source per thread and tile");
+  double px = 0.0, py = 0.0, pz = 0.0, pa = 0.0, pb = 1.0;
+  auto fetch = [&](const int64_t base) {
+    const int64_t last = s1 - 1;
+    const int64_t sidx = min(base + tid, last);
+    px = d.src[3 * sidx];
+    py = d.src[3 * sidx + 1];
+    pz = d.src[3 * sidx + 2];
+    pa = d.srcA != nullptr ? d.srcA[sidx] : 0.0;
+    pb = d.srcB != nullptr ? d.srcB[sidx] : 1.0;
+  };
+  if (s0 < s1) fetch(s0);
+
   R tx[RPT], ty[RPT], tz[RPT], tq[RPT], tqm[RPT];
   double dl[RPT];
+  TgtLoad tl[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int64_t t = t0 + (int64_t)r * TPB + tt;
+    const int64_t tcl = min(t, d.nt - 1);
+    tl[r] = load_offset_inputs<PREC, KIND>(d, tcl);
+    tl[r].x = d.tgt[3 * tcl];
+    tl[r].y = d.tgt[3 * tcl + 1];
+    tl[r].z = d.tgt[3 * tcl + 2];
+  }
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
     const int64_t t = t0 + (int64_t)r * TPB + tt;
     tidx[r] = t;
     const bool ok = t < d.nt;
-    tx[r] = ok ? (R)(kScaled ? d.tgt[3 * t] * csc : d.tgt[3 * t]) : (R)0;
-    ty[r] = ok ? (R)(kScaled ? d.tgt[3 * t + 1] * csc : d.tgt[3 * t + 1]) : (R)0;
-    tz[r] = ok ? (R)(kScaled ? d.tgt[3 * t + 2] * csc : d.tgt[3 * t + 2]) : (R)0;
-    const double q = ok ? target_offset<PREC, KIND>(d, t) : 0.0;
+    tx[r] = ok ? (R)(kScaled ? tl[r].x * csc : tl[r].x) : (R)0;
+    ty[r] = ok ? (R)(kScaled ? tl[r].y * csc : tl[r].y) : (R)0;
+    tz[r] = ok ? (R)(kScaled ? tl[r].z * csc : tl[r].z) : (R)0;
+    const double q = ok ? offset_from<PREC, KIND>(d, tl[r]) : 0.0;
     if (PREC == 1) {
       const float q2 = (float)(q * kLog2e);
       tq[r] = (R)q2;
@@ -542,22 +598,20 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
     const int cnt = (int)min((int64_t)TILE, s1 - base);
     const int cntp = (cnt + kSub * G - 1) / (kSub * G) * (kSub * G);
     group_sync<NTHR>();
-    for (int i = tid; i < cntp; i += NTHR) {
-      if (i < cnt) {
-        const int64_t s = base + i;
-        SrcPt<R> sp;
-        sp.x = (R)(kScaled ? d.src[3 * s] * csc : d.src[3 * s]);
-        sp.y = (R)(kScaled ? d.src[3 * s + 1] * csc : d.src[3 * s + 1]);
-        sp.z = (R)(kScaled ? d.src[3 * s + 2] * csc : d.src[3 * s + 2]);
-        R wsrc;
-        source_terms<PREC, kExpKind>(d, s, &sp.a, &wsrc);
-        s_p[i] = sp;
-        s_w[i] = wsrc;
-      } else {
-        s_p[i] = SrcPt<R>{(R)0, (R)0, (R)0, (R)-INFINITY};
-        s_w[i] = (R)0;
-      }
+    if (tid < cnt) {
+      SrcPt<R> sp;
+      sp.x = (R)(kScaled ? px * csc : px);
+      sp.y = (R)(kScaled ? py * csc : py);
+      sp.z = (R)(kScaled ? pz * csc : pz);
+      R wsrc;
+      source_terms_v<PREC, kExpKind>(d, pa, pb, &sp.a, &wsrc);
+      s_p[tid] = sp;
+      s_w[tid] = wsrc;
+    } else if (tid < cntp) {
+      s_p[tid] = SrcPt<R>{(R)0, (R)0, (R)0, (R)-INFINITY};
+      s_w[tid] = (R)0;
     }
+    if (base + TILE < s1) fetch(base + TILE);
     group_sync<NTHR>();
 
     if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
@@ -647,7 +701,7 @@ __global__ __launch_bounds__(kWideThreads) void pair_pass_wide(const PassDesc di
                                                                const int64_t split_len) {
   using R = typename std::conditional<PREC == 0, double, float>::type;
   constexpr int NF = kind_fields<KIND>();
-  constexpr int TILE = PREC == 1 ? kTile : kTile / 2;  // 80 KiB of LDS records either way
+  constexpr int TILE = 64;  // one staged source per lane and tile
   __shared__ SrcPt<R> s_p[kWideWaves][TILE];
   __shared__ R s_w[kWideWaves][TILE];
   __shared__ double s_part[NF][kWideWaves][kWideTargets];
