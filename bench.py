@@ -20,8 +20,10 @@ Extra fields: solves/s and iterations/s for mixed and for the plain FP64
 single-instance path (host-checked stop every `frequency` iterations), the
 mixed-vs-FP64 error of the transport cost, `batched` (256 cell-similarity
 instances in one batch vs the plain FP64 path one instance at a time),
-`divergence` (Sinkhorn divergence with gradients, 65536 points) and `cindex`
-(cell-similarity C-index over all cell pairs, batched vs plain).
+`divergence` (Sinkhorn divergence with gradients, 65536 points), `cindex`
+(cell-similarity C-index over all cell pairs, batched vs plain) and
+`dense_comparator` (the same passes over a stored cost matrix, streamed from
+HBM, against the on-the-fly pass).
 """
 from __future__ import annotations
 
@@ -162,6 +164,75 @@ def bench_cindex(ctx, a, log) -> dict:
             "speedup_vs_plain_fp64": (npairs / dt) / plain_rate}
 
 
+def _sq_cost_matrix(x, y):
+    import numpy as np
+    W = np.empty((len(x), len(y)))
+    for i0 in range(0, len(x), 512):
+        d = x[i0:i0 + 512, None, :] - y[None, :, :]
+        W[i0:i0 + 512] = (d * d).sum(-1)
+    return W
+
+
+def bench_dense(ctx, a, log) -> dict:
+    """The reference's plain dense GPU path (row LSE and column marginal over
+    a stored cost matrix, MirrorProx.jl:236-333) as the HBM-streaming
+    comparator: the same passes as the on-the-fly kernels on g x g grids
+    (squared Euclidean), the FP32 matrix and its transpose resident in HBM,
+    each pass timed from graph replays; achieved = matrix bytes + vector
+    bytes per pass / time."""
+    import numpy as np
+    import culamp_amd
+    from culamp_amd import MIXED, workloads
+    from culamp_amd import kernels as K
+
+    def timed(fn, reps=20):
+        fn()
+        fn()
+        g = ctx.capture(lambda: [fn() for _ in range(reps)])
+        g.launch()
+        ctx.timer_start()
+        for _ in range(5):
+            g.launch()
+        ms = ctx.timer_stop() / (5 * reps)
+        g.close()
+        return ms * 1e3
+
+    out = []
+    for grid in a.dense_grids:
+        x, y, r, c = workloads.grid_histograms(grid)
+        n = len(x)
+        W = _sq_cost_matrix(x, y)
+        winf = float(W.max())
+        th = np.random.default_rng(0).uniform(-1e-3, 1e-3, n)
+        dev = lambda v: culamp_amd.DeviceArray.from_host(ctx, v)  # noqa: E731
+        dth, dr, dx, dy = dev(th), dev(r), dev(x), dev(y)
+        o, res = culamp_amd.DeviceArray(ctx, n), culamp_amd.DeviceArray(ctx, n)
+        dm = culamp_amd.DenseProblem(W, "mixed", ctx)
+        del W
+        reg = a.eta / 4.0
+        t_lse = timed(lambda: dm.lse_rows(o, dth, reg, 1.0, winf))
+        lz = dev(o.get())
+        t_col = timed(lambda: dm.col_marginal(res, dr, dth, lz, reg, 1.0, winf))
+        t_otf = timed(lambda: K.warp_logsumexp_spp_ct_fused(ctx, o, dx, dy, dth, reg, 1.0, winf,
+                                                            2.0, MIXED))
+        dm.close()
+        stream = n * n * 4 + 3 * 8 * n
+        rec = {"n": n, "m": n, "matrix_bytes": n * n * 4,
+               "dense_lse_us": t_lse, "dense_col_us": t_col, "on_the_fly_lse_us": t_otf,
+               "dense_lse_gbs": stream / (t_lse * 1e-6) / 1e9,
+               "dense_col_gbs": (stream + 8 * n) / (t_col * 1e-6) / 1e9,
+               "dense_lse_frac": stream / (t_lse * 1e-6) / 1e9 / HBM_PEAK_GBS,
+               "on_the_fly_speedup_vs_dense": t_lse / t_otf}
+        log(f"dense comparator n={n}: dense LSE {t_lse:.1f} us ({rec['dense_lse_gbs']:.0f} GB/s), "
+            f"dense column {t_col:.1f} us, on-the-fly LSE {t_otf:.1f} us "
+            f"({rec['on_the_fly_speedup_vs_dense']:.2f}x)")
+        out.append(rec)
+    return {"workload": "row LSE / column marginal over a stored FP32 cost matrix (and its "
+                        "transpose) on g x g grids, squared Euclidean, against the on-the-fly "
+                        "mixed row LSE on the same points",
+            "sizes": out}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_lse_mixed_grid64.json")
 
 
@@ -296,6 +367,10 @@ def main() -> None:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--plain-seconds", type=float, default=60.0,
                     help="time budget of the sequential FP64 single-instance comparison")
+    ap.add_argument("--dense-grids", type=int, nargs="*", default=[64, 128])
+    ap.add_argument("--no-dense", action="store_true")
+    ap.add_argument("--dense-only", action="store_true",
+                    help="only the dense-matrix comparator (profiling)")
     ap.add_argument("--cindex-cells", type=int, default=64)
     ap.add_argument("--cindex-batch", type=int, default=2016)
     ap.add_argument("--no-cindex", action="store_true")
@@ -311,6 +386,10 @@ def main() -> None:
         return
     if a.batch_kernel_only:
         batch_kernel_only(a)
+        return
+    if a.dense_only:
+        import culamp_amd
+        bench_dense(culamp_amd.Context(0), a, lambda *x: print(*x, file=sys.stderr, flush=True))
         return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
@@ -395,6 +474,7 @@ def main() -> None:
     batched = None if a.no_batch else bench_batched(ctx, a, log)
     divergence = None if a.no_diff else bench_divergence(ctx, a, log)
     cidx = None if a.no_cindex else bench_cindex(ctx, a, log)
+    dense = None if a.no_dense else bench_dense(ctx, a, log)
 
     ms_per_step = dt / a.steps * 1e3
     rec = {
@@ -417,6 +497,8 @@ def main() -> None:
         rec["divergence"] = divergence
     if cidx is not None:
         rec["cindex"] = cidx
+    if dense is not None:
+        rec["dense_comparator"] = dense
     print(json.dumps(rec), flush=True)
 
 

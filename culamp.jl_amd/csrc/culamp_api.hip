@@ -508,6 +508,7 @@ int culamp_ctx_destroy(culamp_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   culamp_destroy_batches(ctx);
+  while (!ctx->denses.empty()) culamp_dense_destroy(*ctx->denses.begin());
   while (!ctx->problems.empty()) culamp_problem_destroy(*ctx->problems.begin());
   if (ctx->partial) (void)hipFree(ctx->partial);
   if (ctx->scal) (void)hipFree(ctx->scal);

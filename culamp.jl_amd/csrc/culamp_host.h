@@ -28,6 +28,7 @@ struct culamp_ctx {
   int64_t passes = 0;
   std::set<culamp_problem*> problems;  // destroyed with the context
   std::set<culamp_batch*> batches;     // destroyed with the context (first)
+  std::set<culamp_dense*> denses;      // destroyed with the context
 };
 
 struct culamp_problem {

@@ -21,6 +21,7 @@ from . import diff
 from .cellsim import (FeatureProblem, annealed_sinkhorn_cellsim, cost_scale,
                       extragradient_cellsim, normalize_cell_cost_metric, sinkhorn_cellsim)
 from . import cindex
+from .dense import DenseProblem
 from .cindex import (CellCostKernel, OTScOmicsCellData, compute_otscomics_c_index,
                      streaming_c_index)
 
@@ -32,5 +33,5 @@ __all__ = [
     "annealed_sinkhorn_cellsim", "cost_scale", "extragradient_cellsim",
     "normalize_cell_cost_metric", "sinkhorn_cellsim", "Batch", "divergence_opts",
     "sinkhorn_divergence", "diff", "cindex", "CellCostKernel", "OTScOmicsCellData",
-    "compute_otscomics_c_index", "streaming_c_index",
+    "compute_otscomics_c_index", "streaming_c_index", "DenseProblem",
 ]

@@ -105,6 +105,10 @@ SIGNATURES = {
     "culamp_memcpy_dtod": (_I, [_P, _P, _P, C.c_size_t]),
     "culamp_timer_start": (_I, [_P]),
     "culamp_timer_stop": (_I, [_P, C.POINTER(_D)]),
+    "culamp_dense_create": (_I, [_P, _P, C.c_int64, C.c_int64, _I, C.POINTER(_P)]),
+    "culamp_dense_destroy": (_I, [_P]),
+    "culamp_dense_lse_rows": (_I, [_P, _P, _P, _D, _D, _D, _P]),
+    "culamp_dense_col_marginal": (_I, [_P, _P, _P, _P, _P, _D, _D, _D, _P]),
     "culamp_capture_begin": (_I, [_P]),
     "culamp_capture_end": (_I, [_P, C.POINTER(_P)]),
     "culamp_graph_launch": (_I, [_P, _P]),

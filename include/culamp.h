@@ -69,6 +69,7 @@ typedef struct culamp_ctx culamp_ctx;
 typedef struct culamp_problem culamp_problem;
 typedef struct culamp_batch culamp_batch;
 typedef struct culamp_graph culamp_graph;
+typedef struct culamp_dense culamp_dense;
 
 /* Mirrors EOTArgs (src/Utilities.jl:18-31) field for field, same defaults via
  * culamp_args_default.  Two fields are new and default to the reference's
@@ -433,6 +434,26 @@ int culamp_lse_adjoint(culamp_ctx *ctx, const double *tgt, int64_t nt,
                        const double *tgtA, double gamma, int normalized,
                        const double *tgtW, double c0, double c1, double beta,
                        double *out0, double *out1);
+
+/* ---- dense cost matrix (the plain dense GPU path) -----------------------
+ * The reference's dense-W kernels, kept as the HBM-streaming comparator of
+ * the on-the-fly passes: the same row LSE and column marginal computed from
+ * a stored n x m cost matrix.  culamp_dense_create uploads W (host, n x m
+ * row-major FP64, i.e. the bytes of the reference's column-major Wt) and its
+ * transpose, as FP64 (CULAMP_FP64) or FP32 (CULAMP_MIXED: FP32 costs and
+ * exponents, FP64 sums); the passes take device vectors. */
+int culamp_dense_create(culamp_ctx *ctx, const double *w, int64_t n, int64_t m,
+                        int precision, culamp_dense **out);
+int culamp_dense_destroy(culamp_dense *dm);
+/* warp_logsumexp_fused! (src/solvers/MirrorProx.jl:236-296):
+ * out_i = LSE_j(-(st W_ij / (2 W_inf) + theta_j) / reg), i < n. */
+int culamp_dense_lse_rows(culamp_ctx *ctx, const culamp_dense *dm, const double *theta,
+                          double reg, double st, double winf, double *out);
+/* residual_c! (src/solvers/MirrorProx.jl:299-333):
+ * out_j = sum_i r_i exp(-(st W_ij / (2 W_inf) + theta_j) / reg - logZ_i), j < m. */
+int culamp_dense_col_marginal(culamp_ctx *ctx, const culamp_dense *dm, const double *r,
+                              const double *theta, const double *logZ, double reg,
+                              double st, double winf, double *out);
 
 #ifdef __cplusplus
 }
