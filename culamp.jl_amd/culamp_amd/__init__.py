@@ -22,6 +22,7 @@ from .cellsim import (FeatureProblem, annealed_sinkhorn_cellsim, cost_scale,
                       extragradient_cellsim, normalize_cell_cost_metric, sinkhorn_cellsim)
 from . import cindex
 from .dense import DenseProblem
+from . import dotmark
 from .cindex import (CellCostKernel, OTScOmicsCellData, compute_otscomics_c_index,
                      streaming_c_index)
 
@@ -33,5 +34,5 @@ __all__ = [
     "annealed_sinkhorn_cellsim", "cost_scale", "extragradient_cellsim",
     "normalize_cell_cost_metric", "sinkhorn_cellsim", "Batch", "divergence_opts",
     "sinkhorn_divergence", "diff", "cindex", "CellCostKernel", "OTScOmicsCellData",
-    "compute_otscomics_c_index", "streaming_c_index", "DenseProblem",
+    "compute_otscomics_c_index", "streaming_c_index", "DenseProblem", "dotmark",
 ]

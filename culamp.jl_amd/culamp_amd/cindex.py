@@ -274,6 +274,37 @@ def streaming_c_index(cell_data: OTScOmicsCellData, kernel: CellCostKernel, args
             "Smax": ext.smax, "Nw": nw, "pairs": npairs}
 
 
+def format_summary(result: dict) -> str:
+    """run_cellsim's summary line (src/CmdLineUtils.jl:254), Julia number
+    formatting: "c_index=..,Sw=..,Smin=..,Smax=..,Nw=..,pairs=.."."""
+    from .solvers import julia_float_str
+    f = julia_float_str
+    return (f"c_index={f(result['c_index'])},Sw={f(result['Sw'])},Smin={f(result['Smin'])},"
+            f"Smax={f(result['Smax'])},Nw={int(result['Nw'])},pairs={int(result['pairs'])}")
+
+
+def run_cellsim(file: str, num_features: int, num_subproblems: int, num_cells: int,
+                args: EOTArgs, *, cost: str = "correlation", normalize_features: bool = True,
+                frequency: int = 100, algorithm: str = "lamp", seed: int = 0, output: str = "",
+                batch_size: int = 512, ctx: Context | None = None) -> dict:
+    """run_cellsim (src/CmdLineUtils.jl:239-261): the C-index of an
+    OT-scOmics table, printed (and written to ``output``) as the summary
+    line."""
+    solver = {"lamp": extragradient_cell_distance, "sinkhorn": sinkhorn_cell_distance,
+              "annealed_sinkhorn": annealed_sinkhorn_cell_distance}[algorithm]
+    result = compute_otscomics_c_index(file, num_features, num_subproblems, num_cells, args,
+                                       metric=normalize_cell_cost_metric(cost),
+                                       normalize_features=normalize_features,
+                                       frequency=frequency, solver=solver, seed=seed,
+                                       batch_size=batch_size, ctx=ctx)
+    line = format_summary(result)
+    print(line, flush=True)
+    if output:
+        with open(output, "w") as f:
+            f.write(line + "\n")
+    return result
+
+
 def compute_otscomics_c_index(fpath: str, num_features: int, num_subproblems: int,
                               num_cells: int, args: EOTArgs, *, metric: str = "correlation",
                               normalize_features: bool = True, frequency: int = 100,
@@ -309,4 +340,4 @@ __all__ = ["OTScOmicsCellData", "CellCostKernel", "infer_cell_cluster", "infer_c
            "read_otscomics_cell_data", "fill_normalized_cell", "cell_marginal",
            "within_cluster_pairs", "extragradient_cell_distance", "sinkhorn_cell_distance",
            "annealed_sinkhorn_cell_distance", "pair_distances", "streaming_c_index",
-           "compute_otscomics_c_index", "synthetic_cell_data"]
+           "compute_otscomics_c_index", "synthetic_cell_data", "format_summary", "run_cellsim"]

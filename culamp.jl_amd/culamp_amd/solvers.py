@@ -203,6 +203,28 @@ def _write_vec(path: str, v) -> None:
             f.write(julia_float_str(x) + "\n")
 
 
+def julia_matrix_str(a) -> str:
+    """Julia's string of a Float64 matrix (``"$(A)"``): rows joined by "; ",
+    entries by " ", ";;" closing an n x 1 matrix (Julia >= 1.7)."""
+    a = np.asarray(a, dtype=np.float64)
+    rows = ["; ".join(" ".join(julia_float_str(v) for v in row) for row in a)]
+    tail = ";;" if a.shape[1] == 1 else ""
+    return "[" + rows[0] + tail + "]"
+
+
+def _write_assignments(path: str, a) -> None:
+    """The reference's assignment files (MirrorProxKernel.jl:1171-1184,
+    SinkhornKernel.jl:594-607, AnnealedSinkhorn.jl:158-172): for every linear
+    index k of the 3 x N barycentric-image matrix, one line "<matrix>,<k>",
+    the whole matrix interpolated each time -- 3N lines of O(N) characters,
+    reproduced as written there."""
+    text = julia_matrix_str(np.asarray(a, dtype=np.float64).T)
+    n = np.asarray(a).size
+    with open(path, "w") as f:
+        for k in range(1, n + 1):
+            f.write(f"{text},{k}\n")
+
+
 def extragradient_euclidean(marginal1, marginal2, location1, location2, out1: str, out2: str,
                             outmu: str, args: EOTArgs, frequency: int, p: float, **kw):
     """MirrorProxKernel.jl:1143-1185: solve and write potentials
@@ -213,6 +235,10 @@ def extragradient_euclidean(marginal1, marginal2, location1, location2, out1: st
         _write_vec(outmu + ".spp_col", nu)
         _write_vec(outmu + ".row", phi)
         _write_vec(outmu + ".col", psi)
+    if out1:
+        _write_assignments(out1, a1)
+    if out2:
+        _write_assignments(out2, a2)
     return nu, phi, psi, a1, a2
 
 
@@ -224,6 +250,10 @@ def sinkhorn_euclidean(marginal1, marginal2, location1, location2, out1: str, ou
     if potentials:
         _write_vec(potentials + ".row", phi)
         _write_vec(potentials + ".col", psi)
+    if out1:
+        _write_assignments(out1, a1)
+    if out2:
+        _write_assignments(out2, a2)
     return phi, psi, a1, a2
 
 
@@ -240,5 +270,9 @@ def annealed_sinkhorn_euclidean(marginal1, marginal2, location1, location2, out1
     if potentials:
         _write_vec(potentials + ".row", phi)
         _write_vec(potentials + ".col", psi)
+    if out1:
+        _write_assignments(out1, a1)
+    if out2:
+        _write_assignments(out2, a2)
     out = (phi, psi, info["objective"])
     return out + (info,) if return_info else out
