@@ -215,6 +215,15 @@ def bench_dense(ctx, a, log) -> dict:
         t_col = timed(lambda: dm.col_marginal(res, dr, dth, lz, reg, 1.0, winf))
         t_otf = timed(lambda: K.warp_logsumexp_spp_ct_fused(ctx, o, dx, dy, dth, reg, 1.0, winf,
                                                             2.0, MIXED))
+        # the reference's dense LAMP loop over the same matrix: iterations/s
+        # (4 dense passes per iteration, one check at the first iteration)
+        lamp_args = culamp_amd.EOTArgs(eta_p=a.eta, epsilon=1e-30, B=1.1, itermax=200,
+                                       verbose=False)
+        dm.lamp(r, c, lamp_args.copy(itermax=5), 10 ** 9, plan=False, log=None)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        _, _, dl = dm.lamp(r, c, lamp_args, 10 ** 9, plan=False, log=None)
+        dense_it_s = dl["iterations"] / (time.perf_counter() - t1)
         dm.close()
         stream = n * n * 4 + 3 * 8 * n
         rec = {"n": n, "m": n, "matrix_bytes": n * n * 4,
@@ -222,10 +231,11 @@ def bench_dense(ctx, a, log) -> dict:
                "dense_lse_gbs": stream / (t_lse * 1e-6) / 1e9,
                "dense_col_gbs": (stream + 8 * n) / (t_col * 1e-6) / 1e9,
                "dense_lse_frac": stream / (t_lse * 1e-6) / 1e9 / HBM_PEAK_GBS,
-               "on_the_fly_speedup_vs_dense": t_lse / t_otf}
+               "on_the_fly_speedup_vs_dense": t_lse / t_otf,
+               "dense_lamp_iterations_per_s": dense_it_s}
         log(f"dense comparator n={n}: dense LSE {t_lse:.1f} us ({rec['dense_lse_gbs']:.0f} GB/s), "
             f"dense column {t_col:.1f} us, on-the-fly LSE {t_otf:.1f} us "
-            f"({rec['on_the_fly_speedup_vs_dense']:.2f}x)")
+            f"({rec['on_the_fly_speedup_vs_dense']:.2f}x); dense LAMP {dense_it_s:.0f} it/s")
         out.append(rec)
     return {"workload": "row LSE / column marginal over a stored FP32 cost matrix (and its "
                         "transpose) on g x g grids, squared Euclidean, against the on-the-fly "

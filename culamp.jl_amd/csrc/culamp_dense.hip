@@ -16,6 +16,7 @@
 #include "culamp_pair_kernel.h"
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 struct culamp_dense {
@@ -25,6 +26,7 @@ struct culamp_dense {
   void* w = nullptr;   // n x m row-major
   void* wt = nullptr;  // m x n row-major
   void* terms = nullptr;  // per-column (exponent term, weight) of the current pass
+  double winf = 0.0;      // max |W_ij| (norm(W, Inf) of the reference)
 };
 
 namespace culamp {
@@ -37,20 +39,22 @@ constexpr int kDenseUnroll = 8;   // columns per lane per step: 8 loads in fligh
 // pair kernels' source_terms: FP32 head and folded remainder in mixed
 // precision) so that the row kernel's work per element is one fma, one
 // exponential and one accumulation.
-template <int PREC>
+template <int PREC, bool EXPKIND>
 __global__ void dense_terms(const PassDesc d, void* terms) {
   using R = typename std::conditional<PREC == 0, double, float>::type;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.ns) return;
   R a, w;
-  source_terms_v<PREC, true, R>(d, d.srcA != nullptr ? d.srcA[c] : 0.0,
-                                d.srcB != nullptr ? d.srcB[c] : 1.0, &a, &w);
+  source_terms_v<PREC, EXPKIND, R>(d, d.srcA != nullptr ? d.srcA[c] : 0.0,
+                                   d.srcB != nullptr ? d.srcB[c] : 1.0, &a, &w);
   ((R*)terms)[2 * c] = a;
   ((R*)terms)[2 * c + 1] = w;
 }
 
 // v_rc = k * M_rc + alpha * srcA[c] + gamma * tgtA[r]; KIND_LSE: out_r =
-// LSE_c v_rc; KIND_SUM: out_r = sum_c srcB[c] exp(v_rc).  M is rows x cols
+// LSE_c v_rc; KIND_SUM: out_r = sum_c srcB[c] exp(v_rc) (and, with out1,
+// out1_r = sum_c srcB[c] exp(v_rc) M_rc); KIND_MIN: out_r = min_c v_rc with
+// gamma = 0 (the min-plus of the eta = 0 dual).  M is rows x cols
 // row-major; the PassDesc carries k, gamma/tgtA (as target_offset reads it),
 // nt = rows, ns = cols, out0 and epi; the column terms come from dense_terms.
 // A workgroup of kDenseWaves waves owns kDenseWaves consecutive rows (one per
@@ -85,8 +89,9 @@ __global__ __launch_bounds__(64 * kDenseWaves) void dense_rows(const PassDesc d,
     }
     st.m = empty_max<PREC, R>();
     st.S = st.Sc = st.Sy0 = st.Sy1 = st.Sy2 = 0.0;
-    st.acc = (R)0;
+    st.acc = KIND == KIND_MIN ? (R)INFINITY : (R)0;
   }
+  const bool needc = KIND == KIND_SUM && d.out1 != nullptr;
   const R* Mrow = M + rw * ncols;
   R mv[U];
   auto fetch = [&](const int64_t c0) {
@@ -114,12 +119,51 @@ __global__ __launch_bounds__(64 * kDenseWaves) void dense_rows(const PassDesc d,
       wv[u] = s_w[64 * u + lane];
       dz[u] = (R)0;
     }
-    // (the running max moves every step, not lazily: a lane's columns of one
-    // row span the row's whole exponent range)
-    accumulate<PREC, KIND, U, R, false>(st, v, wv, dz, dz, dz, dz);
+    if constexpr (KIND == KIND_MIN) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (c0 + 64 * u + lane < ncols) st.acc = st.acc < v[u] ? st.acc : v[u];
+    } else if (needc) {
+      accumulate<PREC, KIND, U, R, true>(st, v, wv, cur, dz, dz, dz);
+    } else {
+      // (the running max moves every step, not lazily: a lane's columns of
+      // one row span the row's whole exponent range)
+      accumulate<PREC, KIND, U, R, false>(st, v, wv, dz, dz, dz, dz);
+    }
   }
   combine_lanes<PREC, KIND, 64, R>(st);
   if (lane == 0 && live) epilogue_one<PREC, KIND, R>(d, row, st);
+}
+
+// out_r = sum_c M_rc x_c (FP64 accumulation), one wave per row.
+template <int PREC>
+__global__ __launch_bounds__(kThreads) void dense_gemv(const void* mat, const double* x,
+                                                       double* out, int64_t rows, int64_t cols) {
+  using R = typename std::conditional<PREC == 0, double, float>::type;
+  const R* M = (const R*)mat;
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (r >= rows) return;
+  double acc = 0.0;
+  for (int64_t c = lane; c < cols; c += 64) acc = fma((double)M[r * cols + c], x[c], acc);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) out[r] = acc;
+}
+
+// The plan of a dense LAMP solve: out_rc = r_r exp(k M_rc + alpha x_c - L_r)
+// in FP64 (the reference's r .* softmax(...), MirrorProx.jl:501).
+template <int PREC>
+__global__ void dense_plan(const void* mat, const double* x, const double* L, const double* r,
+                           double k, double alpha, double* out, int64_t rows, int64_t cols) {
+  using R = typename std::conditional<PREC == 0, double, float>::type;
+  const R* M = (const R*)mat;
+  const int64_t total = rows * cols;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = i / cols, col = i - row * cols;
+    out[i] = r[row] * exp(k * (double)M[i] + alpha * x[col] - L[row]);
+  }
 }
 
 template <typename T>
@@ -157,14 +201,19 @@ int run_dense(culamp_ctx* ctx, const culamp_dense* dm, bool transposed, const Pa
   const void* mat = transposed ? dm->wt : dm->w;
   const unsigned tblocks = (unsigned)((d.ns + 255) / 256);
   void* T = dm->terms;
+  const dim3 g(blocks), b(64 * kDenseWaves);
   if (dm->precision == CULAMP_FP64) {
-    hipLaunchKernelGGL(dense_terms<0>, dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
-    if (kind == KIND_LSE) hipLaunchKernelGGL((dense_rows<0, KIND_LSE>), dim3(blocks), dim3(64 * kDenseWaves), 0, ctx->stream, d, mat, T);
-    else hipLaunchKernelGGL((dense_rows<0, KIND_SUM>), dim3(blocks), dim3(64 * kDenseWaves), 0, ctx->stream, d, mat, T);
+    if (kind == KIND_MIN) hipLaunchKernelGGL((dense_terms<0, false>), dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
+    else hipLaunchKernelGGL((dense_terms<0, true>), dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
+    if (kind == KIND_LSE) hipLaunchKernelGGL((dense_rows<0, KIND_LSE>), g, b, 0, ctx->stream, d, mat, T);
+    else if (kind == KIND_SUM) hipLaunchKernelGGL((dense_rows<0, KIND_SUM>), g, b, 0, ctx->stream, d, mat, T);
+    else hipLaunchKernelGGL((dense_rows<0, KIND_MIN>), g, b, 0, ctx->stream, d, mat, T);
   } else {
-    hipLaunchKernelGGL(dense_terms<1>, dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
-    if (kind == KIND_LSE) hipLaunchKernelGGL((dense_rows<1, KIND_LSE>), dim3(blocks), dim3(64 * kDenseWaves), 0, ctx->stream, d, mat, T);
-    else hipLaunchKernelGGL((dense_rows<1, KIND_SUM>), dim3(blocks), dim3(64 * kDenseWaves), 0, ctx->stream, d, mat, T);
+    if (kind == KIND_MIN) hipLaunchKernelGGL((dense_terms<1, false>), dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
+    else hipLaunchKernelGGL((dense_terms<1, true>), dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
+    if (kind == KIND_LSE) hipLaunchKernelGGL((dense_rows<1, KIND_LSE>), g, b, 0, ctx->stream, d, mat, T);
+    else if (kind == KIND_SUM) hipLaunchKernelGGL((dense_rows<1, KIND_SUM>), g, b, 0, ctx->stream, d, mat, T);
+    else hipLaunchKernelGGL((dense_rows<1, KIND_MIN>), g, b, 0, ctx->stream, d, mat, T);
   }
   HIP_TRY(hipGetLastError());
   ctx->pairs += (double)d.nt * (double)d.ns;
@@ -193,6 +242,7 @@ int culamp_dense_create(culamp_ctx* ctx, const double* w, int64_t n, int64_t m, 
   culamp_dense* dm = new culamp_dense();
   dm->ctx = ctx; dm->n = n; dm->m = m; dm->precision = precision;
   const size_t count = (size_t)n * (size_t)m;
+  for (size_t i = 0; i < count; ++i) dm->winf = std::max(dm->winf, std::fabs(w[i]));
   const size_t esz = precision == CULAMP_FP64 ? sizeof(double) : sizeof(float);
   double* stage = nullptr;
   hipError_t e = hipMalloc(&dm->w, count * esz);
@@ -263,6 +313,210 @@ int culamp_dense_col_marginal(culamp_ctx* ctx, const culamp_dense* dm, const dou
   d.tgtA = theta; d.gamma = -1.0 / reg;
   d.epi = EPI_SUM; d.out0 = out;
   return run_dense(ctx, dm, true, d, KIND_SUM);
+}
+
+// LAMP on a stored cost matrix (src/solvers/MirrorProx.jl:397-503): the
+// dense GPU solver of the reference's `run` command.  W is n x m (rows: r,
+// columns: c), W_inf = max |W_ij|, eta_p' = eta_p / (2 W_inf), s = 1; every
+// `frequency` iterations a check over the plan P = r .* softmax(-(W / 2W_inf
+// + nu') / eta_t) (rows) logs infeas = |c - colsum P|_1, the cost of the
+// rounded plan, primalv and dualv, and stops when infeas and primal - dual are
+// below epsilon / 2.  Returns theta and, optionally, the final plan.
+int culamp_dense_lamp_solve(culamp_ctx* ctx, const culamp_dense* dm, const double* r,
+                            const double* c, const culamp_args* a, int64_t frequency,
+                            const double* theta0, culamp_log_fn log_fn, void* user,
+                            double* theta_out, double* plan_out, culamp_result* out) {
+  TRY(check_dense(ctx, dm));
+  if (!r || !c || !a || !theta_out) return fail(CULAMP_ERR_INVALID_ARG, "null argument");
+  if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
+  const int64_t n = dm->n, m = dm->m;
+  const double W = dm->winf;
+  const double pairs0 = ctx->pairs;
+  const int64_t passes0 = ctx->passes;
+  DevBufs B;
+  double *dr, *dc, *theta, *thetab, *nu, *nub, *res, *etamu, *L, *colsum, *colcost, *bvec,
+      *rowb, *rnu, *u, *lse2, *tmp;
+  for (double** v : {&dr, &L, &rowb, &u, &lse2}) TRY(B.alloc(v, n));
+  for (double** v : {&dc, &theta, &thetab, &nu, &nub, &res, &etamu, &colsum, &colcost, &bvec, &rnu})
+    TRY(B.alloc(v, m));
+  TRY(B.alloc(&tmp, std::max(n, m)));
+  HIP_TRY(hipMemcpyAsync(dr, r, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(dc, c, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+  if (theta0) HIP_TRY(hipMemcpyAsync(theta, theta0, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+  else HIP_TRY(launch_fill(theta, 0.0, m, ctx->stream));
+  for (double* v : {nu, nub, thetab}) HIP_TRY(hipMemcpyAsync(v, theta, sizeof(double) * m, hipMemcpyDeviceToDevice, ctx->stream));
+  const double etap = a->eta_p / 2.0 / W;                                      // 409
+  HIP_TRY(launch_etamu(etamu, dc, a->alpha / (double)m, a->tau_mu, m, ctx->stream));  // 417
+  double rlogr = 0.0;
+  for (int64_t i = 0; i < n; ++i) rlogr += r[i] > 0.0 ? r[i] * std::log(r[i]) : 0.0;
+  const double taup = a->tau_p;
+  const double minv = std::tanh(-a->B / 2.0), maxv = std::tanh(a->B / 2.0);
+  const double t0 = now_s();
+
+  // row LSE of -(s W / (2 W_inf) + x_j) / reg, and the column pass over it
+  auto row_lse = [&](const double* x, double reg, double* dst) {
+    PassDesc d = blank();
+    d.nt = n; d.ns = m;
+    d.k = -(1.0 / (2.0 * W)) / reg;
+    d.srcA = x; d.alpha = -1.0 / reg;
+    d.epi = EPI_LSE; d.out0 = dst;
+    return run_dense(ctx, dm, false, d, KIND_LSE);
+  };
+  auto col_pass = [&](const double* x, const double* logZ, const double* wr, double reg,
+                      double* dst, double* dcost) {
+    PassDesc d = blank();
+    d.nt = m; d.ns = n;
+    d.k = -(1.0 / (2.0 * W)) / reg;
+    d.srcA = logZ; d.alpha = -1.0;
+    d.srcB = wr;
+    d.tgtA = x; d.gamma = -1.0 / reg;
+    d.epi = EPI_SUM; d.out0 = dst; d.out1 = dcost;
+    return run_dense(ctx, dm, true, d, KIND_SUM);
+  };
+  auto infeas = [&](const double* x, double reg) {  // 425-428
+    TRY(row_lse(x, reg, L));
+    return col_pass(x, L, dr, reg, res, nullptr);
+  };
+
+  double etat = INFINITY;
+  double primal = NAN, dual = NAN, feas = NAN, objective = NAN;
+  int converged = 0;
+  int64_t steps = 0;
+  if (a->verbose) emit(log_fn, user, "time(s),iter,infeas,ot_objective,primal,dual,pdgap,solver\n");
+  TRY(infeas(nu, etat));                                                       // 436
+  for (int64_t it = 1; it <= a->itermax; ++it) {
+    const double elapsed = now_s() - t0;
+    if (elapsed > a->tmax) break;
+    if ((it - 1) % frequency == 0) {                                           // 444-461
+      TRY(row_lse(nu, etat, L));
+      TRY(col_pass(nu, L, dr, etat, colsum, colcost));
+      ReduceSpec rs{};
+      rs.nops = 4;
+      rs.op[0] = 1; rs.a[0] = dc; rs.b[0] = colsum; rs.len[0] = m;   // |c - colsum|_1
+      rs.op[1] = 0; rs.a[1] = colcost; rs.len[1] = m;                // <W, r .* p>
+      rs.op[2] = 2; rs.a[2] = nu; rs.b[2] = colsum; rs.len[2] = m;   // <nu, colsum>
+      rs.op[3] = 2; rs.a[3] = dr; rs.b[3] = L; rs.len[3] = n;        // <r, L>
+      double v[4];
+      TRY(reduce_to_host(ctx, rs, v));
+      feas = v[0];
+      const double wrp = v[1];
+      // sum_i r_i sum_j p_ij log p_ij with log p_ij = v_ij - L_i and rows of p
+      // summing to one
+      const double rnegent = -(wrp / (2.0 * W) + v[2]) / etat - v[3];
+      primal = 2.0 * (0.5 * wrp + W * feas + etap * rnegent + etap * rlogr);  // primalv, 28-32
+      // dualv (7-13)
+      if (etap == 0.0) {
+        PassDesc d = blank();
+        d.nt = n; d.ns = m;
+        d.k = 1.0;
+        d.srcA = theta; d.alpha = 2.0 * W;
+        d.epi = EPI_MIN; d.out0 = lse2;
+        TRY(run_dense(ctx, dm, false, d, KIND_MIN));
+      } else {
+        TRY(row_lse(theta, etap / W, lse2));
+      }
+      ReduceSpec rd{};
+      rd.nops = 2;
+      rd.op[0] = 2; rd.a[0] = dc; rd.b[0] = theta; rd.len[0] = m;
+      rd.op[1] = 2; rd.a[1] = dr; rd.b[1] = lse2; rd.len[1] = n;
+      double w2[2];
+      TRY(reduce_to_host(ctx, rd, w2));
+      if (etap == 0.0) dual = -2.0 * W * w2[0] + w2[1];
+      else dual = 2.0 * ((-W * w2[0] - etap * w2[1]) + etap * rlogr);
+      // cost of round(r .* p, r, c) (Utilities.jl:158-165): the rows of r .* p
+      // already sum to r (row scaling 1), b_j = min(c_j / colsum_j, 1),
+      // rmu = r - rowsum(P b), rnu = c - b .* colsum, and
+      // <W, round> = <b, colcost> + rmu' W rnu / |rmu|_1
+      {
+        std::vector<double> hs(m), hb(m), hrn(m);
+        TRY(copy_out(ctx, hs.data(), colsum, m));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (int64_t j = 0; j < m; ++j) {
+          hb[j] = std::min(c[j] / hs[j], 1.0);
+          hrn[j] = c[j] - hb[j] * hs[j];
+        }
+        HIP_TRY(hipMemcpyAsync(bvec, hb.data(), sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(rnu, hrn.data(), sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+        // rowsum(P b)_i = r_i sum_j p_ij b_j
+        PassDesc d = blank();
+        d.nt = n; d.ns = m;
+        d.k = -(1.0 / (2.0 * W)) / etat;
+        d.srcA = nu; d.alpha = -1.0 / etat;
+        d.srcB = bvec;
+        d.tgtA = L; d.gamma = -1.0;
+        d.epi = EPI_SUM; d.out0 = rowb;
+        TRY(run_dense(ctx, dm, false, d, KIND_SUM));
+        const unsigned gb = (unsigned)((n + kThreads / 64 - 1) / (kThreads / 64));
+        if (dm->precision == CULAMP_FP64) hipLaunchKernelGGL(dense_gemv<0>, dim3(gb), dim3(kThreads), 0, ctx->stream, dm->w, rnu, u, n, m);
+        else hipLaunchKernelGGL(dense_gemv<1>, dim3(gb), dim3(kThreads), 0, ctx->stream, dm->w, rnu, u, n, m);
+        HIP_TRY(hipGetLastError());
+        std::vector<double> hrb(n), hu(n);
+        TRY(copy_out(ctx, hrb.data(), rowb, n));
+        TRY(copy_out(ctx, hu.data(), u, n));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::vector<double> hcc(m);
+        TRY(copy_out(ctx, hcc.data(), colcost, m));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        double bc = 0.0, num = 0.0, l1 = 0.0;
+        for (int64_t j = 0; j < m; ++j) bc += hb[j] * hcc[j];
+        for (int64_t i = 0; i < n; ++i) {
+          const double rmu = r[i] - r[i] * hrb[i];
+          num += rmu * hu[i];
+          l1 += std::fabs(rmu);
+        }
+        objective = bc + num / l1;
+      }
+      if (a->verbose)
+        emit(log_fn, user, fmt("%.6e", elapsed) + "," + std::to_string(it) + "," +
+                               fmt("%.14e", feas) + "," + fmt("%.14e", objective) + "," +
+                               fmt("%.14e", primal) + "," + fmt("%.14e", dual) + "," +
+                               fmt("%.14e", primal - dual) + ",lamp_cuda\n");
+      if (feas < a->epsilon / 2.0 && primal - dual < a->epsilon / 2.0) {
+        converged = 1;
+        break;
+      }
+    }
+    HIP_TRY(launch_theta_update(thetab, theta, res, dc, etamu, m, a->tau_mu * a->eta_mu, 0, minv,
+                                maxv, 1.0, ctx->stream));                       // 463
+    etat = 1.0 / (taup + (1.0 / etat) * (1.0 - etap));                         // 465
+    HIP_TRY(launch_axpby(nub, theta, taup * etat, nu, 1.0 - taup * etat, m, ctx->stream));  // 467
+    TRY(infeas(nub, etat));                                                    // 475
+    // the reference's update (476), then the clamp (484), in one pass
+    HIP_TRY(launch_theta_update(theta, theta, res, dc, etamu, m, a->eta_mu, 1, minv, maxv, 1.0,
+                                ctx->stream));
+    HIP_TRY(launch_axpby(nu, nu, 1.0 - taup * etat, thetab, taup * etat, m, ctx->stream));  // 477
+    TRY(infeas(nu, etat));                                                     // 487
+    steps += 1;
+  }
+  TRY(copy_out(ctx, theta_out, theta, m));
+  if (plan_out) {                                                              // 501
+    TRY(row_lse(nu, etat, L));
+    DevBufs P;
+    double* dplan;
+    TRY(P.alloc(&dplan, n * m));
+    const double k = -(1.0 / (2.0 * W)) / etat, alpha = -1.0 / etat;
+    if (dm->precision == CULAMP_FP64) hipLaunchKernelGGL(dense_plan<0>, dim3(4096), dim3(256), 0, ctx->stream, dm->w, nu, L, dr, k, alpha, dplan, n, m);
+    else hipLaunchKernelGGL(dense_plan<1>, dim3(4096), dim3(256), 0, ctx->stream, dm->w, nu, L, dr, k, alpha, dplan, n, m);
+    HIP_TRY(hipGetLastError());
+    TRY(copy_out(ctx, plan_out, dplan, n * m));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (out) {
+    out->iterations = steps;
+    out->objective = objective;
+    out->residual = feas;
+    out->primal = primal;
+    out->dual = dual;
+    out->winf = W;
+    out->eta_t = etat;
+    out->elapsed_s = now_s() - t0;
+    out->converged = converged;
+    out->pairs = ctx->pairs - pairs0;
+    out->passes = ctx->passes - passes0;
+    out->nonfinite = !(std::isfinite(feas) && std::isfinite(objective));
+  }
+  return CULAMP_OK;
 }
 
 }  // extern "C"

@@ -109,6 +109,8 @@ SIGNATURES = {
     "culamp_dense_destroy": (_I, [_P]),
     "culamp_dense_lse_rows": (_I, [_P, _P, _P, _D, _D, _D, _P]),
     "culamp_dense_col_marginal": (_I, [_P, _P, _P, _P, _P, _D, _D, _D, _P]),
+    "culamp_dense_lamp_solve": (_I, [_P, _P, _P, _P, C.POINTER(culamp_args), C.c_int64, _P,
+                                     LOG_FN, _P, _P, _P, C.POINTER(culamp_result)]),
     "culamp_capture_begin": (_I, [_P]),
     "culamp_capture_end": (_I, [_P, C.POINTER(_P)]),
     "culamp_graph_launch": (_I, [_P, _P]),

@@ -9,8 +9,9 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import FP64, MIXED, Context, DeviceArray, check, load
-from .solvers import default_context
+from ._lib import (FP64, LOG_FN, MIXED, Context, DeviceArray, check, culamp_result, load, ptr)
+from .args import EOTArgs
+from .solvers import _stdout_log, default_context
 
 
 class DenseProblem:
@@ -54,5 +55,35 @@ class DenseProblem:
                                                logZ.ptr, float(reg), float(st), float(winf),
                                                out.ptr))
 
+    def lamp(self, r, c, args: EOTArgs, frequency: int = 50, theta0=None, plan: bool = True,
+             log=_stdout_log):
+        """LAMP(r, c, W::CuArray, args, frequency; _θ) (MirrorProx.jl:397-503)
+        -> (plan or None, theta, info)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        if r.size != self.n or c.size != self.m:
+            raise ValueError("r and c must match W's rows and columns")
+        t0 = None if theta0 is None else np.ascontiguousarray(theta0, dtype=np.float64)
+        theta = np.empty(self.m)
+        P = np.empty((self.n, self.m)) if plan else None
+        res, a = culamp_result(), args.to_c()
+        cb = LOG_FN(0) if log is None else LOG_FN(lambda _u, line: log(line.decode()))
+        check(load().culamp_dense_lamp_solve(self.ctx.handle, self.handle, ptr(r), ptr(c),
+                                             C.byref(a), int(frequency), ptr(t0), cb, None,
+                                             ptr(theta), ptr(P), C.byref(res)))
+        return P, theta, res.as_dict()
 
-__all__ = ["DenseProblem"]
+
+def dense_lamp(r, c, W, args: EOTArgs, frequency: int = 50, *, theta0=None,
+               precision: str = "fp64", log=_stdout_log, ctx: Context | None = None):
+    """LAMP(r, c, W, args, frequency) on the device (MirrorProx.jl:397-503)
+    -> (r .* p, θ)."""
+    dm = DenseProblem(W, precision, ctx)
+    try:
+        P, theta, _ = dm.lamp(r, c, args, frequency, theta0, True, log)
+    finally:
+        dm.close()
+    return P, theta
+
+
+__all__ = ["DenseProblem", "dense_lamp"]

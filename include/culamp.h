@@ -455,6 +455,18 @@ int culamp_dense_col_marginal(culamp_ctx *ctx, const culamp_dense *dm, const dou
                               const double *theta, const double *logZ, double reg,
                               double st, double winf, double *out);
 
+/* LAMP over the stored matrix, LAMP(r, c, W::CuArray, args, frequency;
+ * _θ) (src/solvers/MirrorProx.jl:397-503): r (n) and c (m) host vectors,
+ * theta0 (m, host) or NULL for zeros; returns theta (m) and, when plan is
+ * not NULL, the final plan r .* softmax(...) (n x m row-major).  Logs, when
+ * args->verbose, the reference's "time(s),iter,infeas,ot_objective,primal,
+ * dual,pdgap,solver" lines; stops when infeas and primal - dual are below
+ * epsilon / 2. */
+int culamp_dense_lamp_solve(culamp_ctx *ctx, const culamp_dense *dm, const double *r,
+                            const double *c, const culamp_args *args, int64_t frequency,
+                            const double *theta0, culamp_log_fn log_fn, void *user,
+                            double *theta, double *plan, culamp_result *out);
+
 #ifdef __cplusplus
 }
 #endif

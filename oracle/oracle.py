@@ -365,3 +365,103 @@ def annealed_sinkhorn_cellsim_solve(data1, data2, r, c, metric="cosine", scale=N
                                                     C.byref(res))
     assert rc == 0
     return dict(phi=phi, psi=psi, log=lg.lines, **res.as_dict())
+
+
+# ---- dense-matrix LAMP (numpy restatement; small sizes) ---------------------
+
+def _softmax_rows(x):
+    """softmax(x; norm_dims=2) (src/Utilities.jl:93-105)."""
+    mx = x.max(axis=1, keepdims=True)
+    v1 = np.exp(x - mx)
+    return v1 / v1.sum(axis=1, keepdims=True)
+
+
+def _lse_rows_np(x):
+    """logsumexp(x, 2) (src/Utilities.jl:34-41)."""
+    mx = x.max(axis=1, keepdims=True)
+    return (np.log(np.exp(x - mx).sum(axis=1, keepdims=True)) + mx)[:, 0]
+
+
+def _round_plan(g, mu, nu):
+    """round(gamma, mu, nu) (src/Utilities.jl:158-165)."""
+    gp = g * np.minimum(mu / g.sum(axis=1), 1.0)[:, None]
+    gpp = gp * np.minimum(nu / gp.sum(axis=0), 1.0)[None, :]
+    rmu = mu - gpp.sum(axis=1)
+    rnu = nu - gpp.sum(axis=0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return gpp + np.outer(rmu, rnu) / np.abs(rmu).sum()
+
+
+def _neg_entropy_rows(p):
+    """neg_entropy(p, dims=2) (src/Utilities.jl:137-143)."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = np.where(p > 0.0, p * np.log(np.where(p > 0.0, p, 1.0)), 0.0)
+    return t.sum(axis=1)
+
+
+def dense_lamp_solve(r, c, W, frequency=50, eta_p=0.0, eta_mu=0.0, tau_p=1.0, tau_mu=1.0,
+                     alpha=0.01, B=1.0, epsilon=1e-4, itermax=10000, verbose=True, theta0=None,
+                     **_ignored):
+    """LAMP(r, c, W::CuArray, args, frequency) (src/solvers/MirrorProx.jl:397-503)
+    restated in numpy, FP64: returns dict(plan, theta, log, iterations).
+    W is len(r) x len(c)."""
+    r, c, W = np.asarray(r, float), np.asarray(c, float), np.asarray(W, float)
+    winf = np.abs(W).max()                                       # 404
+    n = c.size
+    theta = np.zeros(n) if theta0 is None else np.array(theta0, float)
+    etap = eta_p / 2 / winf                                      # 409
+    nu, nub, thetab = theta.copy(), theta.copy(), theta.copy()
+    eta_mu_v = (c + alpha / n) / tau_mu                           # 417
+    minv, maxv = np.tanh(-B / 2), np.tanh(B / 2)
+
+    def infeas(th, eta):                                          # 425-428
+        a = 1.0 / (2 * eta * winf)
+        sumvals = _lse_rows_np(-(a * W + (th / eta)[None, :]))
+        val = -(0.5 * W / winf + th[None, :]) / eta - sumvals[:, None]
+        return (r[:, None] * np.exp(val)).sum(axis=0)
+
+    def update(th0, res, em, adjust):                              # MirrorProx.jl:159-180
+        diff = 2.0 * (res - c) / eta_mu_v
+        mx = np.maximum(diff, -diff)
+        expv = np.exp(diff - mx) * ((th0 + 1) / (1 - th0)) ** (1 - em)
+        out = (expv - np.exp(-mx)) / (expv + np.exp(-mx))
+        return np.clip(out, minv, maxv) if adjust else out
+
+    def primalv(p):                                               # 28-32
+        rp = r[:, None] * p
+        return 2 * (0.5 * (W * rp).sum() + (winf * np.abs(rp.sum(axis=0) - c)).sum()
+                    + etap * r @ _neg_entropy_rows(p) + etap * r @ np.log(r))
+
+    def dualv(th):                                                # 7-13
+        if etap == 0:
+            return -(c @ (2 * winf * th)) + r @ (W + 2 * winf * th[None, :]).min(axis=1)
+        lse = _lse_rows_np(-(0.5 * W + (winf * th)[None, :]) / etap)
+        return 2 * ((-winf * (c @ th) - etap * (r @ lse)) + etap * (r @ np.log(r)))
+
+    etat = np.inf
+    with np.errstate(divide="ignore", invalid="ignore"):
+        res = infeas(nu, etat)
+    log = []
+    it_done = 0
+    for i in range(1, itermax + 1):
+        if (i - 1) % frequency == 0:                              # 444-461
+            with np.errstate(divide="ignore", invalid="ignore"):
+                p = _softmax_rows(-(W / (2 * winf) + nu[None, :]) / etat)
+            obj = (W * _round_plan(r[:, None] * p, r, c)).sum()
+            pv, dv = primalv(p), dualv(theta)
+            feas = np.abs(c - (r[:, None] * p).sum(axis=0)).sum()
+            log.append((i, feas, obj, pv, dv))
+            if feas < epsilon / 2 and pv - dv < epsilon / 2:
+                break
+        thetab = update(theta, res, tau_mu * eta_mu, False)        # 463
+        etat = 1 / (tau_p + (1 / etat) * (1 - etap))              # 465
+        nub = tau_p * etat * theta + (1 - tau_p * etat) * nu       # 467
+        res = infeas(nub, etat)                                    # 475
+        theta = update(theta, res, eta_mu, False)                  # 476
+        nu = (1 - tau_p * etat) * nu + tau_p * etat * thetab       # 477
+        theta = np.clip(theta, minv, maxv)                         # 484
+        res = infeas(nu, etat)                                     # 487
+        it_done = i
+    p = _softmax_rows(-(W / (2 * winf) + nu[None, :]) / etat)     # 501
+    return {"plan": r[:, None] * p, "theta": theta, "log": log, "iterations": it_done,
+            "eta_t": etat, "winf": winf}

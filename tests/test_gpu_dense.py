@@ -108,3 +108,64 @@ def test_dense_grid_4096_matches_on_the_fly(gpu_ctx, oracle):
     ref = oracle.lse_rows(x[rows], y, th, 0.0025, 1.0, winf, 2.0)
     assert np.max(np.abs(a[rows] - ref) / np.maximum(np.abs(ref), 1.0)) < FP64_REL
     dm.close()
+
+
+def _parse(lines):
+    out = []
+    for line in lines:
+        f = line.strip().split(",")
+        if f[0].startswith("time"):
+            continue
+        out.append((int(f[1]), float(f[2]), float(f[3]), float(f[4]), float(f[5]), float(f[6])))
+    return out
+
+
+@pytest.mark.parametrize("eta", [1e-2, 0.0])
+def test_dense_lamp_fp64_vs_oracle(gpu_ctx, oracle, eta):
+    """The dense-matrix LAMP loop (MirrorProx.jl:397-503) on a 10 x 10 grid:
+    every logged check (infeasibility, rounded-plan cost, primal, dual, gap)
+    and the final theta and plan against the oracle's restatement."""
+    import culamp_amd
+    from culamp_amd import workloads
+    x, y, r, c = workloads.uniform_grid_pair(10)
+    W = oracle.grid_cost_matrix(10, 10, 2.0)
+    kw = dict(eta_p=eta, epsilon=1e-9, B=1.1, itermax=400, verbose=True)
+    o = oracle.dense_lamp_solve(r, c, W, frequency=20, **kw)
+    lines = []
+    dm = culamp_amd.DenseProblem(W, "fp64", gpu_ctx)
+    P, theta, info = dm.lamp(r, c, culamp_amd.EOTArgs(**kw), 20, log=lines.append)
+    dm.close()
+    got = _parse(lines)
+    assert len(got) == len(o["log"])
+    for g, (it, feas, obj, pv, dv) in zip(got, o["log"]):
+        assert g[0] == it
+        assert abs(g[1] - feas) <= 1e-9 * max(feas, 1e-6), (it, g[1], feas)
+        assert abs(g[2] - obj) <= 1e-9 * abs(obj), (it, g[2], obj)
+        assert abs(g[3] - pv) <= 1e-9 * abs(pv) and abs(g[4] - dv) <= 1e-9 * abs(dv), it
+    assert info["iterations"] == o["iterations"]
+    assert np.max(np.abs(theta - o["theta"])) < 1e-9
+    assert np.max(np.abs(P - o["plan"])) < 1e-12
+
+
+def test_dense_lamp_ragged_and_mixed(gpu_ctx, oracle):
+    """A non-square random cost (n = 70, m = 45): FP64 against the oracle;
+    the FP32 matrix (mixed) reaches the same plan cost within 1e-5."""
+    import culamp_amd
+    rng = np.random.default_rng(9)
+    n, m = 70, 45
+    W = rng.uniform(0, 1, (n, m)) ** 2
+    r, c = rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(m))
+    kw = dict(eta_p=5e-2, epsilon=1e-9, B=1.1, itermax=600, verbose=False)
+    o = oracle.dense_lamp_solve(r, c, W, frequency=25, **kw)
+    dm = culamp_amd.DenseProblem(W, "fp64", gpu_ctx)
+    P, theta, info = dm.lamp(r, c, culamp_amd.EOTArgs(**kw), 25, log=None)
+    dm.close()
+    assert info["iterations"] == o["iterations"]
+    assert np.max(np.abs(P - o["plan"])) < 1e-12
+    assert np.max(np.abs(theta - o["theta"])) < 1e-9
+    d32 = culamp_amd.DenseProblem(W, "mixed", gpu_ctx)
+    P32, _, _ = d32.lamp(r, c, culamp_amd.EOTArgs(**kw), 25, log=None)
+    d32.close()
+    cost, cost32 = float((W * o["plan"]).sum()), float((W * P32).sum())
+    say("dense lamp mixed cost rel err", abs(cost32 - cost) / cost)
+    assert abs(cost32 - cost) < 1e-5 * cost
