@@ -23,6 +23,8 @@ from .cellsim import (FeatureProblem, annealed_sinkhorn_cellsim, cost_scale,
 from . import cindex
 from .dense import DenseProblem
 from . import dotmark
+from . import images
+from .images import load_rgb, save_image
 from .cindex import (CellCostKernel, OTScOmicsCellData, compute_otscomics_c_index,
                      streaming_c_index)
 
@@ -34,5 +36,6 @@ __all__ = [
     "annealed_sinkhorn_cellsim", "cost_scale", "extragradient_cellsim",
     "normalize_cell_cost_metric", "sinkhorn_cellsim", "Batch", "divergence_opts",
     "sinkhorn_divergence", "diff", "cindex", "CellCostKernel", "OTScOmicsCellData",
-    "compute_otscomics_c_index", "streaming_c_index", "DenseProblem", "dotmark",
+    "compute_otscomics_c_index", "streaming_c_index", "DenseProblem", "dotmark", "images",
+    "load_rgb", "save_image",
 ]
