@@ -150,8 +150,38 @@ __device__ __forceinline__ void combine_lanes(TState<R>& st) {
 }
 
 // Epilogue of one target once its state is complete.
+// The per-target inputs of the SUM / LSE epilogues, loaded together so their
+// round trips overlap (and, in the workgroup-wide layout, ahead of the pair
+// loop).
+struct EpiLoad {
+  double ta;                      // tgtA[t] (mixed SUM exponent correction)
+  double mg;                      // marg[t] (EPI_SK_HALF, EPI_SK_RESID)
+  double th0, c, emu, nu, mix;    // EPI_LAMP_COL fused update inputs
+};
+
+template <int PREC, int KIND>
+__device__ __forceinline__ EpiLoad load_epilogue_inputs(const PassDesc& d, const int64_t t) {
+  EpiLoad e;
+  e.ta = e.mg = e.th0 = e.c = e.emu = e.nu = e.mix = 0.0;
+  if constexpr (KIND == KIND_SUM) {
+    if (PREC == 1 && d.tgtA != nullptr) e.ta = d.tgtA[t];
+    if (d.epi == EPI_SK_RESID) e.mg = d.marg[t];
+    if (d.epi == EPI_LAMP_COL && d.fuse_update) {
+      e.th0 = d.th0[t];
+      e.c = d.cvec[t];
+      e.emu = d.etamu[t];
+      e.nu = d.nu_in[t];
+      e.mix = d.mix_src[t];
+    }
+  } else if constexpr (KIND == KIND_LSE) {
+    if (d.epi == EPI_SK_HALF) e.mg = d.marg[t];
+  }
+  return e;
+}
+
 template <int PREC, int KIND, typename R>
-__device__ __forceinline__ void epilogue_one(const PassDesc& d, const int64_t t, const TState<R>& st) {
+__device__ __forceinline__ void epilogue_from(const PassDesc& d, const int64_t t,
+                                              const TState<R>& st, const EpiLoad& E) {
   if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
     d.out0[t] = (double)st.acc;
   } else if constexpr (KIND == KIND_BARY) {
@@ -184,31 +214,34 @@ __device__ __forceinline__ void epilogue_one(const PassDesc& d, const int64_t t,
     if ((double)st.m <= (double)empty_max<PREC, R>() || st.S == 0.0) lse = -INFINITY;
     else if (PREC == 1) lse = ((double)st.m + log2(st.S) - st.qd) * kLn2;
     else lse = (double)st.m + log(st.S);
-    if (d.epi == EPI_SK_HALF) d.out0[t] = log(d.marg[t]) - lse;
+    if (d.epi == EPI_SK_HALF) d.out0[t] = log(E.mg) - lse;
     else d.out0[t] = lse;
   } else {  // KIND_SUM
     double a = st.S, c = st.Sc;
     if (PREC == 1 && d.tgtA != nullptr) {
-      const double corr = exp2(d.gamma * d.tgtA[t] * kLog2e - st.qd);
+      const double corr = exp2(d.gamma * E.ta * kLog2e - st.qd);
       a *= corr;
       c *= corr;
     }
     if (d.epi == EPI_SK_RESID) {
-      d.out0[t] = d.marg[t] - a;
+      d.out0[t] = E.mg - a;
       if (d.out1 != nullptr) d.out1[t] = c;
     } else {
       d.out0[t] = a;
       if (d.out1 != nullptr) d.out1[t] = c;
       if (d.epi == EPI_LAMP_COL && d.fuse_update) {
-        const double th0 = d.th0[t];
-        const double mixv = d.mix_src[t];
-        const double th = theta_update(a, d.cvec[t], d.etamu[t], th0, d.eta_mu,
-                                       d.adjust, d.minv, d.maxv, d.upd_winf);
+        const double th = theta_update(a, E.c, E.emu, E.th0, d.eta_mu, d.adjust, d.minv,
+                                       d.maxv, d.upd_winf);
         d.th_out[t] = th;
-        d.nu_out[t] = (1.0 - d.mix_t) * d.nu_in[t] + d.mix_t * mixv;
+        d.nu_out[t] = (1.0 - d.mix_t) * E.nu + d.mix_t * E.mix;
       }
     }
   }
+}
+
+template <int PREC, int KIND, typename R>
+__device__ __forceinline__ void epilogue_one(const PassDesc& d, const int64_t t, const TState<R>& st) {
+  epilogue_from<PREC, KIND, R>(d, t, st, load_epilogue_inputs<PREC, KIND>(d, t));
 }
 
 // With one split the workgroup applies the epilogue itself; with several it
@@ -289,6 +322,16 @@ __device__ __forceinline__ void merge_body(const PassDesc& d, const Work& w, con
   const int sub = threadIdx.x % kMergeLanes;
   const int64_t t = block * (kThreads / kMergeLanes) + threadIdx.x / kMergeLanes;
   const bool ok = t < d.nt;
+  // the owner lane's per-target inputs go out with the partial loads
+  const bool own = ok && sub == 0;
+  TgtLoad L{};
+  EpiLoad E{};
+  if constexpr (KIND != KIND_BARY) {
+    if (own) {
+      L = load_offset_inputs<PREC, KIND>(d, t);
+      E = load_epilogue_inputs<PREC, KIND>(d, t);
+    }
+  }
   TState<R> st;
   st.m = empty_max<PREC, R>();
   st.S = st.Sc = st.Sy0 = st.Sy1 = st.Sy2 = 0.0;
@@ -314,10 +357,16 @@ __device__ __forceinline__ void merge_body(const PassDesc& d, const Work& w, con
     }
   }
   combine_lanes<PREC, KIND, kMergeLanes, R>(st);
-  if (!ok || sub != 0) return;
-  const double q = target_offset<PREC, KIND>(d, t);
-  st.qd = PREC == 1 ? (double)(float)(q * kLog2e) : q;
-  epilogue_one<PREC, KIND, R>(d, t, st);
+  if (!own) return;
+  if constexpr (KIND == KIND_BARY) {
+    const double q = target_offset<PREC, KIND>(d, t);
+    st.qd = PREC == 1 ? (double)(float)(q * kLog2e) : q;
+    epilogue_one<PREC, KIND, R>(d, t, st);
+  } else {
+    const double q = offset_from<PREC, KIND>(d, L);
+    st.qd = PREC == 1 ? (double)(float)(q * kLog2e) : q;
+    epilogue_from<PREC, KIND, R>(d, t, st, E);
+  }
 }
 
 template <int PREC, int KIND>
@@ -712,25 +761,35 @@ __global__ __launch_bounds__(kWideThreads) void pair_pass_wide(const PassDesc di
   const int64_t s1 = min(d.ns, s0 + split_len);
   const int nsplit = (int)((d.ns + split_len - 1) / split_len);
   const int64_t none = -1;
+  // The merge lane of each target (below) loads the target's offset and
+  // epilogue inputs now, so that their round trip overlaps the pair loop
+  // instead of following the merge.
+  const int sub = threadIdx.x % kMergeLanes, tl = threadIdx.x / kMergeLanes;
+  const int64_t t = t0 + tl;
+  const bool own = threadIdx.x < kWideTargets * kMergeLanes && sub == 0 && t < d.nt;
+  TgtLoad L{};
+  EpiLoad E{};
+  if (own) {
+    L = load_offset_inputs<PREC, KIND>(d, t);
+    E = load_epilogue_inputs<PREC, KIND>(d, t);
+  }
   int64_t tidx[1];
   TState<R> st[1];
   pair_accum<PREC, KIND, COST, 1, 64, TILE, R>(d, s_p[wave], s_w[wave], lane, t0, s0, s1, none,
                                                 tidx, st);
   if (tidx[0] != none) {
-    const int tl = (int)(tidx[0] - t0);
+    const int tp = (int)(tidx[0] - t0);
     double f[5];
     if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) f[0] = (double)st[0].acc;
     else if constexpr (KIND == KIND_SUM) { f[0] = st[0].S; f[1] = st[0].Sc; }
     else { f[0] = (double)st[0].m; f[1] = st[0].S; f[2] = st[0].Sy0; f[3] = st[0].Sy1; f[4] = st[0].Sy2; }
 #pragma unroll
-    for (int q = 0; q < NF; ++q) s_part[q][wave][tl] = f[q];
+    for (int q = 0; q < NF; ++q) s_part[q][wave][tp] = f[q];
   }
   __syncthreads();
   // merge_body over the LDS partials: kMergeLanes lanes per target, lane
   // `sub` holding split `sub`
   if (threadIdx.x >= kWideTargets * kMergeLanes) return;
-  const int sub = threadIdx.x % kMergeLanes, tl = threadIdx.x / kMergeLanes;
-  const int64_t t = t0 + tl;
   const bool ok = t < d.nt;
   TState<R> acc;
   acc.m = empty_max<PREC, R>();
@@ -755,10 +814,10 @@ __global__ __launch_bounds__(kWideThreads) void pair_pass_wide(const PassDesc di
     combine_state<PREC, KIND, R>(acc, p);
   }
   combine_lanes<PREC, KIND, kMergeLanes, R>(acc);
-  if (!ok || sub != 0) return;
-  const double q = target_offset<PREC, KIND>(d, t);
+  if (!own) return;
+  const double q = offset_from<PREC, KIND>(d, L);
   acc.qd = PREC == 1 ? (double)(float)(q * kLog2e) : q;
-  epilogue_one<PREC, KIND, R>(d, t, acc);
+  epilogue_from<PREC, KIND, R>(d, t, acc, E);
 }
 
 template <int PREC, int KIND, int COST, int RPT>
