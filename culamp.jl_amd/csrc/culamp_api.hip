@@ -539,6 +539,13 @@ int culamp_ctx_set_layout(culamp_ctx* ctx, int layout) {
   return CULAMP_OK;
 }
 
+int culamp_ctx_set_source_records(culamp_ctx* ctx, int enable) {
+  TRY(check_ctx(ctx));
+  if (enable != 0 && enable != 1) return fail(CULAMP_ERR_INVALID_ARG, "enable must be 0 or 1");
+  ctx->records = enable;
+  return CULAMP_OK;
+}
+
 int culamp_ctx_synchronize(culamp_ctx* ctx) {
   TRY(check_ctx(ctx));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1063,43 +1070,105 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
   int converged = 0;
   int64_t steps = 0;
 
+  // Staged source records (mixed precision): each iteration pass's epilogue
+  // writes the FP32 source records of the pass after it (the row LSE for the
+  // column pass over x, the column pass for the next row LSE over y), so
+  // those passes skip converting their sources.  recy_ok: rec_y holds the
+  // records of lse(nu or nub, etat, st) for the current etat and st.
+  const bool use_rec = ctx->records && prec == CULAMP_MIXED;
+  float4 *rec_x = nullptr, *rec_y = nullptr;
+  float *recw_x = nullptr, *recw_y = nullptr;
+  if (use_rec) {
+    double *bx, *by, *wx, *wy;
+    TRY(B.alloc(&bx, 2 * n)); TRY(B.alloc(&by, 2 * m));
+    TRY(B.alloc(&wx, (n + 1) / 2)); TRY(B.alloc(&wy, (m + 1) / 2));
+    rec_x = reinterpret_cast<float4*>(bx); rec_y = reinterpret_cast<float4*>(by);
+    recw_x = reinterpret_cast<float*>(wx); recw_y = reinterpret_cast<float*>(wy);
+  }
+  bool recy_ok = false;
+  // this row LSE's epilogue: records of the column pass over x that follows
+  auto emit_x = [&](PassDesc& d) {
+    if (!use_rec) return;
+    const PassDesc nx = colmarg_desc(pb->x, n, pb->y, m, pb->r, nu, logZ, 1.0, 1.0, winf, p,
+                                     res, cost);
+    d.rec_out = rec_x; d.recw_out = recw_x;
+    d.rec_alpha = nx.alpha; d.rec_k = nx.k; d.rec_scaled = 0; d.rec_wsrc = nx.srcB;
+  };
+  // this column pass's epilogue: records of lse(., reg, s) over y
+  auto emit_y = [&](PassDesc& d, double reg, double s) {
+    if (!use_rec) return;
+    const PassDesc ny = lse_rows_desc(pb->x, n, pb->y, m, nu, reg, s, winf, p, logZ);
+    d.rec_out = rec_y; d.recw_out = recw_y;
+    d.rec_alpha = ny.alpha; d.rec_k = ny.k; d.rec_scaled = cost_of_p(p) == COST_SQL2;
+    d.rec_wsrc = ny.srcB;
+  };
   auto lse = [&](const double* pot, double reg, double s, double* dst, bool offset) {
     PassDesc d = lse_rows_desc(pb->x, n, pb->y, m, pot, reg, s, winf, p, dst);
     d.prev_offset = offset ? 1 : 0;
     return run_pass(ctx, d, prec, KIND_LSE);
   };
+  // the row LSE of the loop: lse(pot, etat, st) -> logZ, reading rec_y when
+  // it is current and writing rec_x
+  auto lse_step = [&](const double* pot, double reg, double s) {
+    PassDesc d = lse_rows_desc(pb->x, n, pb->y, m, pot, reg, s, winf, p, logZ);
+    d.prev_offset = 1;
+    if (use_rec && recy_ok) { d.rec = rec_y; d.recw = recw_y; }
+    emit_x(d);
+    return run_pass(ctx, d, prec, KIND_LSE);
+  };
+  auto use_x = [&](PassDesc& d) {
+    if (use_rec) { d.rec = rec_x; d.recw = recw_x; }
+  };
+  // The column pass of the first half step, fused with the theta update and
+  // the nu mixing (1103-1109); `with_cost` also writes the column costs
+  // (a check needs them).
+  auto col_step1 = [&](double etat_new, double st_new, bool with_cost) {
+    PassDesc d = colmarg_desc(pb->x, n, pb->y, m, pb->r, nu, logZ, etat, st, winf, p, res, cost);
+    d.epi = EPI_LAMP_COL; d.fuse_update = 1; d.adjust = 0;
+    d.th0 = theta; d.cvec = pb->c; d.etamu = etamu; d.eta_mu = a->eta_mu;
+    d.minv = minv; d.maxv = maxv; d.upd_winf = 1.0; d.th_out = thetab;
+    d.nu_in = nu; d.mix_src = theta; d.nu_out = nub; d.mix_t = etat_new;
+    if (!with_cost) d.out1 = nullptr;  // the column costs are only read at checks
+    use_x(d);
+    emit_y(d, etat_new, st_new);  // nub, for the next row LSE
+    return run_pass(ctx, d, prec, KIND_SUM);
+  };
   for (int64_t it = 1; it <= a->itermax; ++it) {
     const double elapsed = now_s() - t0;
     if (elapsed > a->tmax) break;
     const bool check_it = a->verbose || a->stop == CULAMP_STOP_INFEAS;
+    const double etat_new = 1.0 / (taup + (1.0 / etat) * (1.0 - eta));
+    const double st_new = (1.0 - etat_new) * st + etat_new;
+    bool half_done = false;
     if (check_it && (it - 1) % frequency == 0) {  // 1081-1101
-      TRY(lse(nu, etat, st, logZ, true));
-      TRY(run_pass(ctx, colmarg_desc(pb->x, n, pb->y, m, pb->r, nu, logZ, etat, st, winf, p, res, cost), prec, KIND_SUM));
-      ReduceSpec rs{};
-      rs.nops = 4;
-      rs.op[0] = 1; rs.a[0] = res; rs.b[0] = pb->c; rs.len[0] = m;
-      rs.op[1] = 0; rs.a[1] = cost; rs.len[1] = m;
-      rs.op[2] = 2; rs.a[2] = res; rs.b[2] = nu; rs.len[2] = m;
-      rs.op[3] = 2; rs.a[3] = pb->r; rs.b[3] = logZ; rs.len[3] = n;
-      double v[4];
-      TRY(reduce_to_host(ctx, rs, v));
-      resid = v[0];
-      objective = v[1];
-      const double q = eta / etat;
-      primal = objective * (1.0 - q) - q * v[2] + q * v[3] + 2.0 * winf * resid;
-      ReduceSpec rd{};
-      rd.nops = 2;
+      // The check's row LSE and column pass are those of the first half
+      // step (same nu, eta_t and s_t): the column pass also applies the
+      // step's fused update (to thetab / nub, so theta and nu are unchanged
+      // if the loop stops here) and the step skips both.
+      TRY(lse_step(nu, etat, st));
+      TRY(col_step1(etat_new, st_new, true));
+      half_done = true;
       if (eta > 0) {
         TRY(lse(theta, eta, 1.0, tmp, false));
       } else {
         TRY(run_pass(ctx, minplus_desc(pb->x, n, pb->y, m, theta, winf, p, tmp), CULAMP_FP64, KIND_MIN));
       }
-      rd.op[0] = 2; rd.a[0] = pb->r; rd.b[0] = tmp; rd.len[0] = n;
-      rd.op[1] = 2; rd.a[1] = pb->c; rd.b[1] = theta; rd.len[1] = m;
-      double w[2];
-      TRY(reduce_to_host(ctx, rd, w));
-      if (eta > 0) dual = -2.0 * eta * w[0] - 2.0 * winf * w[1] + 2.0 * hr;
-      else dual = w[0] - 2.0 * winf * w[1];
+      ReduceSpec rs{};
+      rs.nops = 6;
+      rs.op[0] = 1; rs.a[0] = res; rs.b[0] = pb->c; rs.len[0] = m;
+      rs.op[1] = 0; rs.a[1] = cost; rs.len[1] = m;
+      rs.op[2] = 2; rs.a[2] = res; rs.b[2] = nu; rs.len[2] = m;
+      rs.op[3] = 2; rs.a[3] = pb->r; rs.b[3] = logZ; rs.len[3] = n;
+      rs.op[4] = 2; rs.a[4] = pb->r; rs.b[4] = tmp; rs.len[4] = n;
+      rs.op[5] = 2; rs.a[5] = pb->c; rs.b[5] = theta; rs.len[5] = m;
+      double v[6];
+      TRY(reduce_to_host(ctx, rs, v));
+      resid = v[0];
+      objective = v[1];
+      const double q = eta / etat;
+      primal = objective * (1.0 - q) - q * v[2] + q * v[3] + 2.0 * winf * resid;
+      if (eta > 0) dual = -2.0 * eta * v[4] - 2.0 * winf * v[5] + 2.0 * hr;
+      else dual = v[4] - 2.0 * winf * v[5];
       if (a->verbose)
         emit(log_fn, user, fmt("%.6e", elapsed) + "," + std::to_string(it) + "," +
                                fmt("%.14e", resid) + "," + fmt("%.14e", objective) + "," +
@@ -1114,20 +1183,14 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
     }
     // extragradient step (1103-1115); the theta update and the nu mixing are
     // fused into the column-marginal pass epilogue.
-    const double etat_new = 1.0 / (taup + (1.0 / etat) * (1.0 - eta));
-    TRY(lse(nu, etat, st, logZ, true));
-    {
-      PassDesc d = colmarg_desc(pb->x, n, pb->y, m, pb->r, nu, logZ, etat, st, winf, p, res, cost);
-      d.epi = EPI_LAMP_COL; d.fuse_update = 1; d.adjust = 0;
-      d.th0 = theta; d.cvec = pb->c; d.etamu = etamu; d.eta_mu = a->eta_mu;
-      d.minv = minv; d.maxv = maxv; d.upd_winf = 1.0; d.th_out = thetab;
-      d.nu_in = nu; d.mix_src = theta; d.nu_out = nub; d.mix_t = etat_new;
-      d.out1 = nullptr;  // the column costs are only read at checks
-      TRY(run_pass(ctx, d, prec, KIND_SUM));
+    if (!half_done) {
+      TRY(lse_step(nu, etat, st));
+      TRY(col_step1(etat_new, st_new, false));
     }
     etat = etat_new;
-    st = (1.0 - etat) * st + etat;
-    TRY(lse(nub, etat, st, logZ, true));
+    st = st_new;
+    recy_ok = use_rec;
+    TRY(lse_step(nub, etat, st));
     {
       PassDesc d = colmarg_desc(pb->x, n, pb->y, m, pb->r, nub, logZ, etat, st, winf, p, res, cost);
       d.epi = EPI_LAMP_COL; d.fuse_update = 1; d.adjust = 1;
@@ -1135,12 +1198,19 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
       d.minv = minv; d.maxv = maxv; d.upd_winf = 1.0; d.th_out = theta;
       d.nu_in = nu; d.mix_src = thetab; d.nu_out = nu; d.mix_t = etat;
       d.out1 = nullptr;
+      use_x(d);
+      emit_y(d, etat, st);  // nu, for the next iteration's row LSE
       TRY(run_pass(ctx, d, prec, KIND_SUM));
     }
     steps += 1;
   }
   // epilogue (1124-1130)
-  TRY(lse(nu, etat, st, logZ, true));
+  {
+    PassDesc d = lse_rows_desc(pb->x, n, pb->y, m, nu, etat, st, winf, p, logZ);
+    d.prev_offset = 1;
+    if (use_rec && recy_ok) { d.rec = rec_y; d.recw = recw_y; }
+    TRY(run_pass(ctx, d, prec, KIND_LSE));
+  }
   double* img_dev = nullptr;
   if (img1 || img2) TRY(B.alloc(&img_dev, 3 * (n > m ? n : m)));
   if (img1) {

@@ -24,6 +24,7 @@ struct culamp_ctx {
   double* scal_host = nullptr; // pinned mirror
   int force_rpt = 0, force_splits = 0;
   int layout = 0;              // culamp::Layout of point passes
+  int records = 1;             // solver loops pass staged source records between passes
   double pairs = 0.0;          // pair evaluations issued (throughput accounting)
   int64_t passes = 0;
   std::set<culamp_problem*> problems;  // destroyed with the context

@@ -96,6 +96,23 @@ struct PassDesc {
   const double* tgtW;
   double adj_c0, adj_c1, adj_beta;
   int adj_norm;
+  // Staged source records (mixed-precision LSE / SUM passes of the solver
+  // loops).  With `rec` set the pass takes each source's FP32 record
+  // {x, y, z, a} and weight from rec / recw instead of converting src / srcA /
+  // srcB itself; the records hold exactly what that conversion gives.  An
+  // epilogue with rec_out set writes the records of the next pass, whose
+  // sources are this pass's targets, with srcA = the value the epilogue
+  // writes (out0 of an LSE pass, nu_out of EPI_LAMP_COL), exponent factor
+  // rec_alpha, weights rec_wsrc (null: 1) and, when rec_scaled (a mixed
+  // squared-Euclidean LSE consumer), coordinates scaled by sqrt(-rec_k log2 e)
+  // where rec_k is the consumer's k.
+  const float4* rec;
+  const float* recw;
+  float4* rec_out;
+  float* recw_out;
+  double rec_alpha, rec_k;
+  int rec_scaled;
+  const double* rec_wsrc;
 };
 
 enum DynKind : int { DYN_NONE = 0, DYN_LSE = 1, DYN_COL = 2 };
@@ -221,31 +238,57 @@ hipError_t launch_batch_mfma(const BatchLaunch& b, int kind, const Work& w, hipS
 // Body of the one-block reductions: 1024 threads, strided partial sums, then
 // a fixed tree, so the result depends only on the input.
 __device__ __forceinline__ void reduce_block(const ReduceSpec& spec, double* out) {
-  __shared__ double sh[1024];
-  const int tid = threadIdx.x;
-  for (int k = 0; k < spec.nops; ++k) {
+  // Every thread folds its strided elements of each operation, then the
+  // partials meet by butterfly within each wave and across the 16 waves
+  // through LDS: one barrier for all operations.
+  __shared__ double sh[8][16];
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  double acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    acc[k] = (k < spec.nops && spec.op[k] == 3) ? -INFINITY : 0.0;
+    if (k >= spec.nops) continue;
     const int op = spec.op[k];
     const double* a = spec.a[k];
     const double* b = spec.b[k];
     const int64_t n = spec.len[k];
-    double acc = op == 3 ? -INFINITY : 0.0;
+    double v = acc[k];
     for (int64_t i = tid; i < n; i += 1024) {
       const double x = a[i];
-      if (op == 0) acc += x;
-      else if (op == 1) acc += fabs(x - b[i]);
-      else if (op == 2) acc += x * b[i];
-      else if (op == 3) acc = fmax(acc, x);
-      else if (op == 4) { if (x > 0.0) acc += x * log(x); }
-      else acc += fabs(x);
+      if (op == 0) v += x;
+      else if (op == 1) v += fabs(x - b[i]);
+      else if (op == 2) v += x * b[i];
+      else if (op == 3) v = fmax(v, x);
+      else if (op == 4) { if (x > 0.0) v += x * log(x); }
+      else v += fabs(x);
     }
-    sh[tid] = acc;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-      if (tid < s) sh[tid] = op == 3 ? fmax(sh[tid], sh[tid + s]) : sh[tid] + sh[tid + s];
-      __syncthreads();
+    acc[k] = v;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (k >= spec.nops) continue;
+    const bool mx = spec.op[k] == 3;
+    double v = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(v, off);
+      v = mx ? fmax(v, o) : v + o;
     }
-    if (tid == 0) out[k] = sh[0];
-    __syncthreads();
+    if (lane == 0) sh[k][wave] = v;
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (k >= spec.nops) continue;
+    const bool mx = spec.op[k] == 3;
+    double v = lane < 16 ? sh[k][lane] : (mx ? -INFINITY : 0.0);
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+      const double o = __shfl_xor(v, off);
+      v = mx ? fmax(v, o) : v + o;
+    }
+    if (lane == 0) out[k] = v;
   }
 }
 

@@ -157,12 +157,23 @@ struct EpiLoad {
   double ta;                      // tgtA[t] (mixed SUM exponent correction)
   double mg;                      // marg[t] (EPI_SK_HALF, EPI_SK_RESID)
   double th0, c, emu, nu, mix;    // EPI_LAMP_COL fused update inputs
+  double rx, ry, rz, rw;          // record of the next pass: tgt[t], rec_wsrc[t]
 };
 
 template <int PREC, int KIND>
 __device__ __forceinline__ EpiLoad load_epilogue_inputs(const PassDesc& d, const int64_t t) {
   EpiLoad e;
   e.ta = e.mg = e.th0 = e.c = e.emu = e.nu = e.mix = 0.0;
+  e.rx = e.ry = e.rz = 0.0;
+  e.rw = 1.0;
+  if constexpr (PREC == 1 && (KIND == KIND_SUM || KIND == KIND_LSE)) {
+    if (d.rec_out != nullptr) {
+      e.rx = d.tgt[3 * t];
+      e.ry = d.tgt[3 * t + 1];
+      e.rz = d.tgt[3 * t + 2];
+      if (d.rec_wsrc != nullptr) e.rw = d.rec_wsrc[t];
+    }
+  }
   if constexpr (KIND == KIND_SUM) {
     if (PREC == 1 && d.tgtA != nullptr) e.ta = d.tgtA[t];
     if (d.epi == EPI_SK_RESID) e.mg = d.marg[t];
@@ -177,6 +188,42 @@ __device__ __forceinline__ EpiLoad load_epilogue_inputs(const PassDesc& d, const
     if (d.epi == EPI_SK_HALF) e.mg = d.marg[t];
   }
   return e;
+}
+
+// Mixed precision: the FP32 exponent term and weight of a source whose
+// exponent is av (natural units) and weight bw: a = fl32(av log2 e) and
+// w = bw (1 + lo ln 2) with lo the FP64 remainder of a, so that
+// 2^a w = bw e^av to FP32 accuracy.  (Contraction off: the staging in the
+// pair loop and the record written by an epilogue must round alike.)
+__device__ __forceinline__ void split_term(const double av, const double bw, float* a, float* w) {
+#pragma clang fp contract(off)
+  const double a2 = av * kLog2e;
+  const float hi = (float)a2;
+  const double lo = a2 - (double)hi;
+  *a = hi;
+  *w = (float)(isfinite(lo) ? bw * (1.0 + lo * kLn2) : bw);
+}
+
+// Coordinate scale of a mixed squared-Euclidean LSE pass with cost factor k
+// (natural units, k <= 0): sqrt(-k log2 e).
+__device__ __forceinline__ double lse_coord_scale(const double k) {
+#pragma clang fp contract(off)
+  return sqrt(fmax(-(k * kLog2e), 0.0));
+}
+
+// Record of target t as a source of the next pass (see PassDesc::rec); v is
+// the value this epilogue wrote for t.
+__device__ __forceinline__ void emit_record(const PassDesc& d, const int64_t t, const double v,
+                                            const EpiLoad& E) {
+  const double csc = d.rec_scaled ? lse_coord_scale(d.rec_k) : 1.0;
+  float4 r;
+  r.x = (float)(d.rec_scaled ? E.rx * csc : E.rx);
+  r.y = (float)(d.rec_scaled ? E.ry * csc : E.ry);
+  r.z = (float)(d.rec_scaled ? E.rz * csc : E.rz);
+  float w;
+  split_term(d.rec_alpha * v, E.rw, &r.w, &w);
+  d.rec_out[t] = r;
+  d.recw_out[t] = w;
 }
 
 template <int PREC, int KIND, typename R>
@@ -214,8 +261,9 @@ __device__ __forceinline__ void epilogue_from(const PassDesc& d, const int64_t t
     if ((double)st.m <= (double)empty_max<PREC, R>() || st.S == 0.0) lse = -INFINITY;
     else if (PREC == 1) lse = ((double)st.m + log2(st.S) - st.qd) * kLn2;
     else lse = (double)st.m + log(st.S);
-    if (d.epi == EPI_SK_HALF) d.out0[t] = log(E.mg) - lse;
-    else d.out0[t] = lse;
+    const double v = d.epi == EPI_SK_HALF ? log(E.mg) - lse : lse;
+    d.out0[t] = v;
+    if (PREC == 1 && d.rec_out != nullptr) emit_record(d, t, v, E);
   } else {  // KIND_SUM
     double a = st.S, c = st.Sc;
     if (PREC == 1 && d.tgtA != nullptr) {
@@ -233,7 +281,9 @@ __device__ __forceinline__ void epilogue_from(const PassDesc& d, const int64_t t
         const double th = theta_update(a, E.c, E.emu, E.th0, d.eta_mu, d.adjust, d.minv,
                                        d.maxv, d.upd_winf);
         d.th_out[t] = th;
-        d.nu_out[t] = (1.0 - d.mix_t) * E.nu + d.mix_t * E.mix;
+        const double nu = (1.0 - d.mix_t) * E.nu + d.mix_t * E.mix;
+        d.nu_out[t] = nu;
+        if (PREC == 1 && d.rec_out != nullptr) emit_record(d, t, nu, E);
       }
     }
   }
@@ -407,11 +457,10 @@ __device__ __forceinline__ void source_terms_v(const PassDesc& d, const double a
   const double av = d.srcA != nullptr ? d.alpha * a_raw : 0.0;
   const double bw = d.srcB != nullptr ? b_raw : 1.0;
   if (PREC == 1 && EXPKIND) {
-    const double a2 = av * kLog2e;
-    const float hi = (float)a2;
-    const double lo = a2 - (double)hi;
-    *a_out = (R)hi;
-    *w_out = (R)(isfinite(lo) ? bw * (1.0 + lo * kLn2) : bw);
+    float a, w;
+    split_term(av, bw, &a, &w);
+    *a_out = (R)a;
+    *w_out = (R)w;
   } else {
     *a_out = (R)av;
     *w_out = (R)bw;
@@ -589,16 +638,25 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
   // (a_s + tqm) - |t' - s'|^2 is three fused multiply-adds after the
   // differences, with no separate scaling of the cost
   constexpr bool kScaled = kLazy && COST == COST_SQL2;
-  const double csc = kScaled ? sqrt(fmax(-(d.k * kLog2e), 0.0)) : 1.0;
+  const double csc = kScaled ? lse_coord_scale(d.k) : 1.0;
+  // staged records from the previous pass's epilogue (mixed LSE / SUM)
+  const bool use_rec = PREC == 1 && kExpKind && d.rec != nullptr;
   // This thread's source of the next tile, held in registers from its load
   // until it is written to LDS: the loads of a tile go out together and one
   // tile ahead of the pair loop (every index is clamped into range, so no
   // load waits on a branch).
   static_assert(TILE == NTHR, "one staged source per thread and tile");
   double px = 0.0, py = 0.0, pz = 0.0, pa = 0.0, pb = 1.0;
+  float4 rq = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float rqw = 0.0f;
   auto fetch = [&](const int64_t base) {
     const int64_t last = s1 - 1;
     const int64_t sidx = min(base + tid, last);
+    if (use_rec) {
+      rq = d.rec[sidx];
+      rqw = d.recw[sidx];
+      return;
+    }
     px = d.src[3 * sidx];
     py = d.src[3 * sidx + 1];
     pz = d.src[3 * sidx + 2];
@@ -649,11 +707,19 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
     group_sync<NTHR>();
     if (tid < cnt) {
       SrcPt<R> sp;
-      sp.x = (R)(kScaled ? px * csc : px);
-      sp.y = (R)(kScaled ? py * csc : py);
-      sp.z = (R)(kScaled ? pz * csc : pz);
       R wsrc;
-      source_terms_v<PREC, kExpKind>(d, pa, pb, &sp.a, &wsrc);
+      if (use_rec) {
+        sp.x = (R)rq.x;
+        sp.y = (R)rq.y;
+        sp.z = (R)rq.z;
+        sp.a = (R)rq.w;
+        wsrc = (R)rqw;
+      } else {
+        sp.x = (R)(kScaled ? px * csc : px);
+        sp.y = (R)(kScaled ? py * csc : py);
+        sp.z = (R)(kScaled ? pz * csc : pz);
+        source_terms_v<PREC, kExpKind>(d, pa, pb, &sp.a, &wsrc);
+      }
       s_p[tid] = sp;
       s_w[tid] = wsrc;
     } else if (tid < cntp) {

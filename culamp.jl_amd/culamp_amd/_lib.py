@@ -98,6 +98,7 @@ SIGNATURES = {
     "culamp_ctx_synchronize": (_I, [_P]),
     "culamp_ctx_set_launch": (_I, [_P, _I, _I]),
     "culamp_ctx_set_layout": (_I, [_P, _I]),
+    "culamp_ctx_set_source_records": (_I, [_P, _I]),
     "culamp_malloc": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
     "culamp_free": (_I, [_P, _P]),
     "culamp_memcpy_htod": (_I, [_P, _P, _P, C.c_size_t]),
@@ -254,6 +255,12 @@ class Context:
         workgroup, merged in LDS)."""
         codes = {"auto": 0, "split": 1, "wide": 2}
         check(self.lib.culamp_ctx_set_layout(self.handle, codes[layout]))
+
+    def set_source_records(self, enable: bool = True) -> None:
+        """Whether the mixed-precision LAMP loop passes staged FP32 source
+        records from each pass's epilogue to the next pass (identical results
+        either way)."""
+        check(self.lib.culamp_ctx_set_source_records(self.handle, 1 if enable else 0))
 
     def capture(self, fn) -> "Graph":
         """Records the kernel-level calls made by fn() into a replayable graph."""

@@ -127,6 +127,12 @@ int culamp_ctx_set_launch(culamp_ctx *ctx, int rows_per_thread, int splits);
  * results.  With layout 2 a nonzero `splits` of culamp_ctx_set_launch caps
  * the in-workgroup split count. */
 int culamp_ctx_set_layout(culamp_ctx *ctx, int layout);
+/* Staged source records (no reference counterpart; tests and tuning): with
+ * 1 (the default) the mixed-precision LAMP loop has each pass's epilogue
+ * write the FP32 source records (scaled coordinates, exponent head and
+ * weight) of the pass after it, which then skips converting its sources; 0
+ * makes every pass convert its own.  Both give identical results. */
+int culamp_ctx_set_source_records(culamp_ctx *ctx, int enable);
 /* Device memory helpers (stream-ordered on the context's stream). */
 int culamp_malloc(culamp_ctx *ctx, size_t bytes, void **dptr);
 int culamp_free(culamp_ctx *ctx, void *dptr);

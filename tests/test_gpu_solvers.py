@@ -160,3 +160,37 @@ def test_mixed_precision_solvers(gpu_ctx, oracle):
     say("lamp mixed", linfo["iterations"], linfo["residual"], "fp64", lo["iterations"], lo["residual"])
     assert linfo["converged"] == 1 and linfo["residual"] <= 1e-6
     assert abs(linfo["objective"] - lo["objective"]) / lo["objective"] < 1e-5
+
+
+@pytest.mark.parametrize("layout,p,eta", [("auto", 2.0, 1e-2), ("split", 1.0, 0.0),
+                                          ("wide", math.inf, 5e-2), ("split", 2.0, 0.0)])
+def test_lamp_source_records_identical(gpu_ctx, layout, p, eta):
+    """Mixed-precision LAMP with the staged source records passed between
+    passes gives bit-identical logs and outputs to every pass converting its
+    own sources (64 x 64 grid pair and a ragged cloud pair)."""
+    import culamp_amd
+    from culamp_amd import workloads
+    say("lamp source records", layout, p, eta)
+    rng = np.random.default_rng(21)
+    cases = [workloads.grid_histograms(64),
+             (rng.uniform(size=(700, 3)), rng.uniform(size=(450, 3)), hist(22, 700), hist(23, 450))]
+    args = culamp_amd.EOTArgs(eta_p=eta, epsilon=1e-6, B=1.1, itermax=300, verbose=True,
+                              precision="mixed")
+    gpu_ctx.set_layout(layout)
+    try:
+        for x, y, r, c in cases:
+            pb = culamp_amd.Problem(x, y, r, c, gpu_ctx)
+            out = {}
+            for rec in (True, False):
+                gpu_ctx.set_source_records(rec)
+                lines = []
+                res = pb.lamp(args, 25, p, lines.append)
+                out[rec] = ([l.split(",", 1)[1] for l in lines[1:]], res)
+            pb.close()
+            assert out[True][0] == out[False][0]
+            for a, b in zip(out[True][1][:5], out[False][1][:5]):
+                assert np.array_equal(a, b)
+            assert out[True][1][5]["iterations"] == out[False][1][5]["iterations"]
+    finally:
+        gpu_ctx.set_source_records(True)
+        gpu_ctx.set_layout("auto")
