@@ -368,17 +368,46 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
   const double p_lse = (p == INFINITY) ? sk_aux_p(p) : p;
   const double p_aux = sk_aux_p(p);
   const char* tag = cellsim ? ",sinkhorn_cellsim\n" : ",sinkhorn_kernel\n";
+  // Staged source records (mixed point problems): each half step's epilogue
+  // writes the FP32 source records of the other half step (see PassDesc).
+  const bool use_rec = ctx->records && a->precision == CULAMP_MIXED && !cellsim;
+  DevBufs RB;
+  float4 *rec_x = nullptr, *rec_y = nullptr;
+  float *recw_x = nullptr, *recw_y = nullptr;
+  if (use_rec) {
+    double *bx, *by, *wx, *wy;
+    TRY(RB.alloc(&bx, 2 * n)); TRY(RB.alloc(&by, 2 * m));
+    TRY(RB.alloc(&wx, (n + 1) / 2)); TRY(RB.alloc(&wy, (m + 1) / 2));
+    rec_x = reinterpret_cast<float4*>(bx); rec_y = reinterpret_cast<float4*>(by);
+    recw_x = reinterpret_cast<float*>(wx); recw_y = reinterpret_cast<float*>(wy);
+  }
+  bool recy_ok = false;  // rec_y holds the records of psi
+  const int rec_scaled = cost_of_p(p_lse) == COST_SQL2;
   auto half_rows = [&]() {
     PassDesc d1 = sk_half_desc(pb->x, n, pb->y, m, pb->r, st.psi, eta, winf, p_lse, st.phi);
     feat(d1, pb, true);
     d1.prev_offset = 1;
+    if (use_rec) {
+      const PassDesc nx = sk_half_desc(pb->y, m, pb->x, n, pb->c, st.phi, eta, winf, p_lse, st.psi);
+      if (recy_ok) { d1.rec = rec_y; d1.recw = recw_y; }
+      d1.rec_out = rec_x; d1.recw_out = recw_x;
+      d1.rec_alpha = nx.alpha; d1.rec_k = nx.k; d1.rec_scaled = rec_scaled; d1.rec_wsrc = nx.srcB;
+    }
     return run_pass(ctx, d1, a->precision, KIND_LSE);
   };
   auto half_cols = [&]() {
     PassDesc d2 = sk_half_desc(pb->y, m, pb->x, n, pb->c, st.phi, eta, winf, p_lse, st.psi);
     feat(d2, pb, false);
     d2.prev_offset = 1;
-    return run_pass(ctx, d2, a->precision, KIND_LSE);
+    if (use_rec) {
+      const PassDesc ny = sk_half_desc(pb->x, n, pb->y, m, pb->r, st.psi, eta, winf, p_lse, st.phi);
+      d2.rec = rec_x; d2.recw = recw_x;
+      d2.rec_out = rec_y; d2.recw_out = recw_y;
+      d2.rec_alpha = ny.alpha; d2.rec_k = ny.k; d2.rec_scaled = rec_scaled; d2.rec_wsrc = ny.srcB;
+    }
+    const int rc = run_pass(ctx, d2, a->precision, KIND_LSE);
+    recy_ok = use_rec;
+    return rc;
   };
   auto resid = [&]() {
     PassDesc d3 = sk_resid_desc(pb->x, n, pb->y, m, pb->r, st.phi, st.psi, eta, winf, p_aux, st.res, st.cost);

@@ -194,3 +194,35 @@ def test_lamp_source_records_identical(gpu_ctx, layout, p, eta):
     finally:
         gpu_ctx.set_source_records(True)
         gpu_ctx.set_layout("auto")
+
+
+@pytest.mark.parametrize("p", [2.0, 1.0, math.inf])
+def test_sinkhorn_source_records_identical(gpu_ctx, p):
+    """Mixed-precision Sinkhorn and annealed Sinkhorn with staged source
+    records passed between the half steps: bit-identical to every pass
+    converting its own sources (ragged point clouds)."""
+    import culamp_amd
+    say("sinkhorn source records", p)
+    rng = np.random.default_rng(31)
+    x, y = rng.uniform(size=(900, 3)), rng.uniform(size=(610, 3))
+    r, c = hist(32, 900), hist(33, 610)
+    pb = culamp_amd.Problem(x, y, r, c, gpu_ctx)
+    args = culamp_amd.EOTArgs(eta_p=2e-2, epsilon=1e-7, itermax=400, verbose=True,
+                              precision="mixed")
+    out = {}
+    try:
+        for rec in (True, False):
+            gpu_ctx.set_source_records(rec)
+            l1, l2 = [], []
+            sk = pb.sinkhorn(args, 10, p, log=l1.append)
+            an = pb.annealed_sinkhorn(args.copy(eta_p=0.5, anneal_mult=0.5, epsilon=2e-2), 10, p,
+                                      log=l2.append)
+            out[rec] = ([l.split(",", 1)[1] for l in l1[1:]], [l.split(",", 1)[1] for l in l2[1:]],
+                        sk, an)
+    finally:
+        gpu_ctx.set_source_records(True)
+        pb.close()
+    assert out[True][0] == out[False][0] and out[True][1] == out[False][1]
+    for k in (2, 3):
+        for a, b in zip(out[True][k][:4], out[False][k][:4]):
+            assert np.array_equal(a, b)
