@@ -279,6 +279,59 @@ def bench_divergence(ctx, a, log) -> dict:
             "grad_a_centred_norm": float(np.linalg.norm(ga - ga.mean()))}
 
 
+def bench_clouds(ctx, a, log) -> dict:
+    """Colour-transfer-shaped clouds: n = m = 16384 RGB points, 5- and
+    8-component Gaussian mixtures, uniform weights, squared Euclidean,
+    eta_p = 1e-2; Sinkhorn to L1 infeasibility 1e-6 (checked every
+    `frequency` iterations) and a fixed number of LAMP iterations (with
+    uniform marginals LAMP's first check, taken before any step at
+    eta_t = inf, is already feasible, so its infeasibility stop would end the
+    solve at once), mixed precision against the plain FP64 path."""
+    import culamp_amd
+    from culamp_amd import workloads
+    x, y, r, c = workloads.gmm_clouds(a.clouds_n, a.clouds_n)
+    n, m = len(x), len(y)
+    pb = culamp_amd.Problem(x, y, r, c, ctx)
+    base = culamp_amd.EOTArgs(eta_p=1e-2, epsilon=6e-6, B=1.1, itermax=20000, verbose=False,
+                              stop="infeas")
+    out = {"workload": f"two Gaussian-mixture RGB clouds (5 / 8 components), n = m = {n}, "
+                       "uniform weights, squared Euclidean, eta_p = 1e-2; Sinkhorn to L1 "
+                       f"infeasibility 1e-6, LAMP {a.clouds_lamp_iters} iterations"}
+    for solver in ("sinkhorn", "lamp"):
+        rec = {}
+        for prec in ("mixed", "fp64"):
+            args = base.copy(precision=prec)
+            if solver == "lamp":
+                args = args.copy(stop="reference", itermax=a.clouds_lamp_iters)
+            fn = (lambda: pb.sinkhorn(args, a.frequency, 2.0, log=None, assignments=False)[-1]) \
+                if solver == "sinkhorn" else \
+                (lambda: pb.lamp(args, a.frequency, 2.0, log=None, assignments=False)[-1])
+            if prec == "mixed":
+                fn()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            info = fn()
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            b_pair = 4 if prec == "mixed" else 8
+            gbs = (info["pairs"] * b_pair + info["passes"] * pass_bytes(n, m, 0)) / 1e9 / dt
+            rec[prec] = {"seconds": dt, "iterations": info["iterations"],
+                         "iterations_per_s": info["iterations"] / dt,
+                         "residual": info["residual"], "objective": info["objective"],
+                         "converged": info["converged"], "dense_eq_GBps": gbs}
+        rec["speedup_mixed_vs_fp64"] = rec["fp64"]["seconds"] / rec["mixed"]["seconds"]
+        # (the fixed-count LAMP runs take no checks, so they report no objective)
+        om, of = rec["mixed"]["objective"], rec["fp64"]["objective"]
+        rec["objective_rel_diff"] = abs(om - of) / abs(of) if of == of and om == om else None
+        log(f"clouds n={n} {solver}: mixed {rec['mixed']['seconds'] * 1e3:.1f} ms "
+            f"({rec['mixed']['iterations']} it, {rec['mixed']['dense_eq_GBps']:.0f} GB/s dense-eq), "
+            f"fp64 {rec['fp64']['seconds'] * 1e3:.1f} ms ({rec['fp64']['iterations']} it); "
+            f"objective rel diff {rec['objective_rel_diff']}")
+        out[solver] = rec
+    pb.close()
+    return out
+
+
 def lse_roofline(ctx, x, y, theta, a, log) -> dict:
     """Roofline of the dominant kernel: the mixed row LSE (pair_pass + split
     merge, culamp_lse_rows) timed with HIP events on the library's stream."""
@@ -384,6 +437,9 @@ def main() -> None:
     ap.add_argument("--cindex-cells", type=int, default=64)
     ap.add_argument("--cindex-batch", type=int, default=2016)
     ap.add_argument("--no-cindex", action="store_true")
+    ap.add_argument("--clouds-n", type=int, default=16384)
+    ap.add_argument("--no-clouds", action="store_true")
+    ap.add_argument("--clouds-lamp-iters", type=int, default=200)
     ap.add_argument("--diff-n", type=int, default=65536)
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
@@ -485,6 +541,7 @@ def main() -> None:
     divergence = None if a.no_diff else bench_divergence(ctx, a, log)
     cidx = None if a.no_cindex else bench_cindex(ctx, a, log)
     dense = None if a.no_dense else bench_dense(ctx, a, log)
+    clouds = None if a.no_clouds else bench_clouds(ctx, a, log)
 
     ms_per_step = dt / a.steps * 1e3
     rec = {
@@ -509,6 +566,8 @@ def main() -> None:
         rec["cindex"] = cidx
     if dense is not None:
         rec["dense_comparator"] = dense
+    if clouds is not None:
+        rec["clouds"] = clouds
     print(json.dumps(rec), flush=True)
 
 
