@@ -1207,6 +1207,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
                            : (primal - dual < a->epsilon / 6.0 && resid < a->epsilon / 6.0);
       if (met) {
         converged = 1;
+        recy_ok = false;  // rec_y now holds the records of nub, not of nu
         break;
       }
     }

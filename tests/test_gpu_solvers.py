@@ -162,20 +162,26 @@ def test_mixed_precision_solvers(gpu_ctx, oracle):
     assert abs(linfo["objective"] - lo["objective"]) / lo["objective"] < 1e-5
 
 
-@pytest.mark.parametrize("layout,p,eta", [("auto", 2.0, 1e-2), ("split", 1.0, 0.0),
-                                          ("wide", math.inf, 5e-2), ("split", 2.0, 0.0)])
-def test_lamp_source_records_identical(gpu_ctx, layout, p, eta):
+@pytest.mark.parametrize("layout,p,eta,stop", [("auto", 2.0, 1e-2, "reference"),
+                                               ("split", 1.0, 0.0, "reference"),
+                                               ("wide", math.inf, 5e-2, "reference"),
+                                               ("split", 2.0, 0.0, "reference"),
+                                               ("auto", 2.0, 1e-2, "infeas"),
+                                               ("split", 1.0, 5e-2, "infeas")])
+def test_lamp_source_records_identical(gpu_ctx, layout, p, eta, stop):
     """Mixed-precision LAMP with the staged source records passed between
     passes gives bit-identical logs and outputs to every pass converting its
-    own sources (64 x 64 grid pair and a ragged cloud pair)."""
+    own sources (64 x 64 grid pair and a ragged cloud pair), also when the
+    infeasibility stop ends the loop at a check."""
     import culamp_amd
     from culamp_amd import workloads
-    say("lamp source records", layout, p, eta)
+    say("lamp source records", layout, p, eta, stop)
     rng = np.random.default_rng(21)
     cases = [workloads.grid_histograms(64),
              (rng.uniform(size=(700, 3)), rng.uniform(size=(450, 3)), hist(22, 700), hist(23, 450))]
-    args = culamp_amd.EOTArgs(eta_p=eta, epsilon=1e-6, B=1.1, itermax=300, verbose=True,
-                              precision="mixed")
+    args = culamp_amd.EOTArgs(eta_p=eta, epsilon=1e-6 if stop == "reference" else 6e-6, B=1.1,
+                              itermax=300 if stop == "reference" else 20000, verbose=True,
+                              precision="mixed", stop=stop)
     gpu_ctx.set_layout(layout)
     try:
         for x, y, r, c in cases:
@@ -191,6 +197,8 @@ def test_lamp_source_records_identical(gpu_ctx, layout, p, eta):
             for a, b in zip(out[True][1][:5], out[False][1][:5]):
                 assert np.array_equal(a, b)
             assert out[True][1][5]["iterations"] == out[False][1][5]["iterations"]
+            if stop == "infeas":
+                assert out[True][1][5]["converged"] == 1
     finally:
         gpu_ctx.set_source_records(True)
         gpu_ctx.set_layout("auto")
