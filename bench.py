@@ -396,6 +396,29 @@ def kernel_only(a) -> None:
     log(f"copy calibration: 10 x {nbytes} B device-to-device")
 
 
+def solve_only(a) -> None:
+    """Only the headline LAMP solves (mixed, 64 x 64 grids), warm-up and
+    timed, so that a profiler run sees the solve's kernels alone."""
+    import culamp_amd
+    from culamp_amd import workloads
+
+    ctx = culamp_amd.Context(0)
+    x, y, r, c = workloads.grid_histograms(a.grid)
+    pb = culamp_amd.Problem(x, y, r, c, ctx)
+    mixed = culamp_amd.EOTArgs(eta_p=a.eta, epsilon=6e-6, B=1.1, itermax=200000, verbose=False,
+                               stop="infeas", precision="mixed")
+    for _ in range(a.warmup):
+        pb.lamp(mixed, a.frequency, 2.0, log=None, assignments=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        info = pb.lamp(mixed, a.frequency, 2.0, log=None, assignments=False)[-1]
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    print(f"lamp solve: {dt * 1e3:.2f} ms, {info['iterations']} iterations, {info['passes']} passes, "
+          f"infeasibility {info['residual']:.3e}", file=sys.stderr, flush=True)
+
+
 def batch_kernel_only(a) -> None:
     """One batched Sinkhorn solve of the cell-similarity batch, iterations
     launched one by one (no graph) so that a profiler sees every dispatch."""
@@ -444,6 +467,8 @@ def main() -> None:
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="launch only the roofline kernel and a copy of known size (profiling)")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="only the headline LAMP solves (profiling)")
     ap.add_argument("--batch-kernel-only", action="store_true",
                     help="one batched cell-similarity Sinkhorn solve without graphs (profiling)")
     a = ap.parse_args()
@@ -452,6 +477,9 @@ def main() -> None:
         return
     if a.batch_kernel_only:
         batch_kernel_only(a)
+        return
+    if a.solve_only:
+        solve_only(a)
         return
     if a.dense_only:
         import culamp_amd

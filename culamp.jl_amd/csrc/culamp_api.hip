@@ -306,14 +306,11 @@ int device_winf(culamp_ctx* ctx, const double* x, int64_t n, const double* y,
   TRY(run_pass(ctx, max_desc(x, n, y, m, p, tmp_n), CULAMP_FP64, KIND_MAX));
   ReduceSpec rs{};
   rs.nops = 1; rs.op[0] = 3; rs.a[0] = tmp_n; rs.len[0] = n;
-  HIP_TRY(launch_reduce(rs, ctx->scal, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  *winf = ctx->scal_host[0];
+  double v[1];
+  TRY(reduce_to_host(ctx, rs, v));
+  *winf = v[0];
   return CULAMP_OK;
 }
-
-int reduce_to_host(culamp_ctx* ctx, const ReduceSpec& rs, double* out);
 
 // W_inf of a problem: point clouds as max_logsumexp_spp_ct!
 // (MirrorProxKernel.jl:280-326); feature problems as max_logsumexp_cellsim!
@@ -331,12 +328,24 @@ int problem_winf(culamp_ctx* ctx, culamp_problem* pb, double p, double* tmp_n, d
   return CULAMP_OK;
 }
 
-int reduce_to_host(culamp_ctx* ctx, const ReduceSpec& rs, double* out) {
-  HIP_TRY(launch_reduce(rs, ctx->scal, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(double) * rs.nops, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  for (int k = 0; k < rs.nops; ++k) out[k] = ctx->scal_host[k];
+// The reduction writes its scalars straight into coherent pinned memory:
+// reduce_start enqueues it and records ctx->ev_chk, reduce_finish waits for
+// that event and reads them.
+int reduce_start(culamp_ctx* ctx, const ReduceSpec& rs) {
+  HIP_TRY(launch_reduce(rs, ctx->chk_dev, ctx->stream));
+  HIP_TRY(hipEventRecord(ctx->ev_chk, ctx->stream));
   return CULAMP_OK;
+}
+
+int reduce_finish(culamp_ctx* ctx, int nops, double* out) {
+  HIP_TRY(hipEventSynchronize(ctx->ev_chk));
+  for (int k = 0; k < nops; ++k) out[k] = ((volatile double*)ctx->chk_host)[k];
+  return CULAMP_OK;
+}
+
+int reduce_to_host(culamp_ctx* ctx, const ReduceSpec& rs, double* out) {
+  TRY(reduce_start(ctx, rs));
+  return reduce_finish(ctx, rs.nops, out);
 }
 
 int copy_out(culamp_ctx* ctx, double* host, const double* dev, int64_t n) {
@@ -524,6 +533,10 @@ int culamp_ctx_create(int device, culamp_ctx** out) {
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
   if (e == hipSuccess) e = hipMalloc(&ctx->scal, 64 * sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc(&ctx->scal_host, 64 * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_chk, hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = hipHostMalloc(&ctx->chk_host, 64 * sizeof(double), hipHostMallocCoherent | hipHostMallocMapped);
+  if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&ctx->chk_dev, ctx->chk_host, 0);
   if (e != hipSuccess) {
     culamp_ctx_destroy(ctx);
     return fail(CULAMP_ERR_HIP, "context creation failed: %s", hipGetErrorString(e));
@@ -544,6 +557,8 @@ int culamp_ctx_destroy(culamp_ctx* ctx) {
   if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->ev_chk) (void)hipEventDestroy(ctx->ev_chk);
+  if (ctx->chk_host) (void)hipHostFree(ctx->chk_host);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CULAMP_OK;
@@ -1077,10 +1092,12 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
   const double pairs0 = ctx->pairs;
   const int64_t passes0 = ctx->passes;
   DevBufs B;
-  double *theta, *nu, *thetab, *nub, *res, *cost, *etamu, *logZ, *tmp;
+  double *theta, *nu, *thetab, *nub, *res, *cost, *etamu, *logZ, *tmp, *nu_keep, *theta_keep;
   TRY(B.alloc(&theta, m)); TRY(B.alloc(&nu, m)); TRY(B.alloc(&thetab, m));
   TRY(B.alloc(&nub, m)); TRY(B.alloc(&res, m)); TRY(B.alloc(&cost, m));
   TRY(B.alloc(&etamu, m)); TRY(B.alloc(&logZ, n)); TRY(B.alloc(&tmp, n > m ? n : m));
+  double* logZ_keep;
+  TRY(B.alloc(&nu_keep, m)); TRY(B.alloc(&theta_keep, m)); TRY(B.alloc(&logZ_keep, n));
   for (double* v : {theta, nu, thetab, nub, res, cost}) HIP_TRY(launch_fill(v, 0.0, m, ctx->stream));
   HIP_TRY(launch_fill(logZ, 0.0, n, ctx->stream));
   const double t0 = now_s();
@@ -1151,6 +1168,9 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
   // The column pass of the first half step, fused with the theta update and
   // the nu mixing (1103-1109); `with_cost` also writes the column costs
   // (a check needs them).
+  // The dual (and its pass) is needed for the log and the reference's gap
+  // stop; stopping on infeasibility without logging needs neither.
+  const bool need_dual = a->verbose || a->stop != CULAMP_STOP_INFEAS;
   auto col_step1 = [&](double etat_new, double st_new, bool with_cost) {
     PassDesc d = colmarg_desc(pb->x, n, pb->y, m, pb->r, nu, logZ, etat, st, winf, p, res, cost);
     d.epi = EPI_LAMP_COL; d.fuse_update = 1; d.adjust = 0;
@@ -1168,7 +1188,8 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
     const bool check_it = a->verbose || a->stop == CULAMP_STOP_INFEAS;
     const double etat_new = 1.0 / (taup + (1.0 / etat) * (1.0 - eta));
     const double st_new = (1.0 - etat_new) * st + etat_new;
-    bool half_done = false;
+    bool half_done = false, pending = false;
+    const double etat_prev = etat, st_prev = st;
     if (check_it && (it - 1) % frequency == 0) {  // 1081-1101
       // The check's row LSE and column pass are those of the first half
       // step (same nu, eta_t and s_t): the column pass also applies the
@@ -1177,39 +1198,28 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
       TRY(lse_step(nu, etat, st));
       TRY(col_step1(etat_new, st_new, true));
       half_done = true;
-      if (eta > 0) {
-        TRY(lse(theta, eta, 1.0, tmp, false));
-      } else {
-        TRY(run_pass(ctx, minplus_desc(pb->x, n, pb->y, m, theta, winf, p, tmp), CULAMP_FP64, KIND_MIN));
+      if (need_dual) {
+        if (eta > 0) {
+          TRY(lse(theta, eta, 1.0, tmp, false));
+        } else {
+          TRY(run_pass(ctx, minplus_desc(pb->x, n, pb->y, m, theta, winf, p, tmp), CULAMP_FP64, KIND_MIN));
+        }
       }
       ReduceSpec rs{};
-      rs.nops = 6;
+      rs.nops = need_dual ? 6 : 4;
       rs.op[0] = 1; rs.a[0] = res; rs.b[0] = pb->c; rs.len[0] = m;
       rs.op[1] = 0; rs.a[1] = cost; rs.len[1] = m;
       rs.op[2] = 2; rs.a[2] = res; rs.b[2] = nu; rs.len[2] = m;
       rs.op[3] = 2; rs.a[3] = pb->r; rs.b[3] = logZ; rs.len[3] = n;
       rs.op[4] = 2; rs.a[4] = pb->r; rs.b[4] = tmp; rs.len[4] = n;
       rs.op[5] = 2; rs.a[5] = pb->c; rs.b[5] = theta; rs.len[5] = m;
-      double v[6];
-      TRY(reduce_to_host(ctx, rs, v));
-      resid = v[0];
-      objective = v[1];
-      const double q = eta / etat;
-      primal = objective * (1.0 - q) - q * v[2] + q * v[3] + 2.0 * winf * resid;
-      if (eta > 0) dual = -2.0 * eta * v[4] - 2.0 * winf * v[5] + 2.0 * hr;
-      else dual = v[4] - 2.0 * winf * v[5];
-      if (a->verbose)
-        emit(log_fn, user, fmt("%.6e", elapsed) + "," + std::to_string(it) + "," +
-                               fmt("%.14e", resid) + "," + fmt("%.14e", objective) + "," +
-                               fmt("%.14e", primal) + "," + fmt("%.14e", dual) + ",lamp_kernel\n");
-      const bool met = a->stop == CULAMP_STOP_INFEAS
-                           ? resid < a->epsilon / 6.0
-                           : (primal - dual < a->epsilon / 6.0 && resid < a->epsilon / 6.0);
-      if (met) {
-        converged = 1;
-        recy_ok = false;  // rec_y now holds the records of nub, not of nu
-        break;
-      }
+      TRY(reduce_start(ctx, rs));
+      // The stop test is read back after the second half step is enqueued,
+      // so the GPU runs that half step while the host waits; nu, theta and
+      // logZ (the next row LSE's centring offsets) are kept here so that a
+      // stop can restore them.
+      HIP_TRY(launch_copy3(nu_keep, nu, theta_keep, theta, m, logZ_keep, logZ, n, ctx->stream));
+      pending = true;
     }
     // extragradient step (1103-1115); the theta update and the nu mixing are
     // fused into the column-marginal pass epilogue.
@@ -1231,6 +1241,33 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
       use_x(d);
       emit_y(d, etat, st);  // nu, for the next iteration's row LSE
       TRY(run_pass(ctx, d, prec, KIND_SUM));
+    }
+    if (pending) {
+      double v[6];
+      TRY(reduce_finish(ctx, need_dual ? 6 : 4, v));
+      resid = v[0];
+      objective = v[1];
+      const double q = eta / etat_prev;
+      primal = objective * (1.0 - q) - q * v[2] + q * v[3] + 2.0 * winf * resid;
+      if (!need_dual) dual = NAN;
+      else if (eta > 0) dual = -2.0 * eta * v[4] - 2.0 * winf * v[5] + 2.0 * hr;
+      else dual = v[4] - 2.0 * winf * v[5];
+      if (a->verbose)
+        emit(log_fn, user, fmt("%.6e", elapsed) + "," + std::to_string(it) + "," +
+                               fmt("%.14e", resid) + "," + fmt("%.14e", objective) + "," +
+                               fmt("%.14e", primal) + "," + fmt("%.14e", dual) + ",lamp_kernel\n");
+      const bool met = a->stop == CULAMP_STOP_INFEAS
+                           ? resid < a->epsilon / 6.0
+                           : (primal - dual < a->epsilon / 6.0 && resid < a->epsilon / 6.0);
+      if (met) {
+        // undo the half step run past the stop: the state is the check's
+        converged = 1;
+        HIP_TRY(launch_copy3(nu, nu_keep, theta, theta_keep, m, logZ, logZ_keep, n, ctx->stream));
+        etat = etat_prev;
+        st = st_prev;
+        recy_ok = false;  // rec_y describes the undone state
+        break;
+      }
     }
     steps += 1;
   }

@@ -64,7 +64,7 @@ struct Seg {
   double s0;
 };
 
-__global__ __launch_bounds__(1024) void batch_reduce_kernel(const ReduceSpec* specs,
+__global__ __launch_bounds__(kReduceThreads) void batch_reduce_kernel(const ReduceSpec* specs,
                                                             const int32_t* act, double* out) {
   const int32_t k = act[blockIdx.x];
   reduce_block(specs[k], out + 8 * (int64_t)k);
@@ -387,7 +387,7 @@ struct Engine {
 
   int reduce(const ReduceSpec* d_specs, double* d_out) {
     if (active.empty()) return CULAMP_OK;
-    hipLaunchKernelGGL(batch_reduce_kernel, dim3((unsigned)active.size()), dim3(1024), 0,
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3((unsigned)active.size()), dim3(kReduceThreads), 0,
                        ctx->stream, d_specs, d_active.p, d_out);
     HIP_TRY(hipGetLastError());
     return CULAMP_OK;

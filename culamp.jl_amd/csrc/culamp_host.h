@@ -18,6 +18,9 @@ struct culamp_ctx {
   int num_cu = 256;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_chk = nullptr;   // end of a solver check's reduction
+  double* chk_host = nullptr;    // coherent pinned scalars the reductions write directly
+  double* chk_dev = nullptr;     // device address of chk_host
   double* partial = nullptr;
   size_t partial_cap = 0;  // doubles
   double* scal = nullptr;      // device scalars for reductions
@@ -85,6 +88,8 @@ void feat(PassDesc& d, const culamp_problem* pb, bool rows);
 int check_ctx(culamp_ctx* ctx);
 int need_features(const culamp_problem* pb);
 int reduce_to_host(culamp_ctx* ctx, const culamp::ReduceSpec& rs, double* out);
+int reduce_start(culamp_ctx* ctx, const culamp::ReduceSpec& rs);
+int reduce_finish(culamp_ctx* ctx, int nops, double* out);
 int copy_out(culamp_ctx* ctx, double* host, const double* dev, int64_t n);
 double sk_aux_p(double p);
 

@@ -237,38 +237,54 @@ hipError_t launch_batch_mfma(const BatchLaunch& b, int kind, const Work& w, hipS
 
 // Body of the one-block reductions: 1024 threads, strided partial sums, then
 // a fixed tree, so the result depends only on the input.
+// Threads of a reduction workgroup.
+constexpr int kReduceThreads = 1024;
+
+// One element's contribution to reduction `op` (see ReduceSpec).
+__device__ __forceinline__ double reduce_fold(const int op, const double acc, const double x,
+                                              const double y) {
+  if (op == 0) return acc + x;
+  if (op == 1) return acc + fabs(x - y);
+  if (op == 2) return acc + x * y;
+  if (op == 3) return fmax(acc, x);
+  if (op == 4) return x > 0.0 ? acc + x * log(x) : acc;
+  return acc + fabs(x);
+}
+
 __device__ __forceinline__ void reduce_block(const ReduceSpec& spec, double* out) {
-  // Every thread folds its strided elements of each operation, then the
-  // partials meet by butterfly within each wave and across the 16 waves
-  // through LDS: one barrier for all operations.
-  __shared__ double sh[8][16];
+  // One operation at a time (a compact loop: the kernel runs once per check,
+  // mostly from a cold instruction cache), each thread folding four strided
+  // elements per round with their loads in flight together; the partials
+  // meet by butterfly within each wave and across the waves through LDS, one
+  // barrier for all operations.
+  constexpr int W = kReduceThreads / 64;
+  constexpr int U = 4;
+  __shared__ double sh[8][W];
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-  double acc[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    acc[k] = (k < spec.nops && spec.op[k] == 3) ? -INFINITY : 0.0;
-    if (k >= spec.nops) continue;
+  for (int k = 0; k < spec.nops; ++k) {
     const int op = spec.op[k];
     const double* a = spec.a[k];
     const double* b = spec.b[k];
     const int64_t n = spec.len[k];
-    double v = acc[k];
-    for (int64_t i = tid; i < n; i += 1024) {
-      const double x = a[i];
-      if (op == 0) v += x;
-      else if (op == 1) v += fabs(x - b[i]);
-      else if (op == 2) v += x * b[i];
-      else if (op == 3) v = fmax(v, x);
-      else if (op == 4) { if (x > 0.0) v += x * log(x); }
-      else v += fabs(x);
-    }
-    acc[k] = v;
-  }
+    const bool two = op == 1 || op == 2;
+    double v = op == 3 ? -INFINITY : 0.0;
+    for (int64_t i0 = tid; i0 < n; i0 += U * kReduceThreads) {
+      double xa[U], xb[U];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    if (k >= spec.nops) continue;
-    const bool mx = spec.op[k] == 3;
-    double v = acc[k];
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = i0 + (int64_t)u * kReduceThreads;
+        xa[u] = 0.0;
+        xb[u] = 0.0;
+        if (i < n) {
+          xa[u] = a[i];
+          if (two) xb[u] = b[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i0 + (int64_t)u * kReduceThreads < n) v = reduce_fold(op, v, xa[u], xb[u]);
+    }
+    const bool mx = op == 3;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       const double o = __shfl_xor(v, off);
@@ -278,13 +294,11 @@ __device__ __forceinline__ void reduce_block(const ReduceSpec& spec, double* out
   }
   __syncthreads();
   if (wave != 0) return;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    if (k >= spec.nops) continue;
+  for (int k = 0; k < spec.nops; ++k) {
     const bool mx = spec.op[k] == 3;
-    double v = lane < 16 ? sh[k][lane] : (mx ? -INFINITY : 0.0);
+    double v = lane < W ? sh[k][lane] : (mx ? -INFINITY : 0.0);
 #pragma unroll
-    for (int off = 8; off > 0; off >>= 1) {
+    for (int off = W / 2; off > 0; off >>= 1) {
       const double o = __shfl_xor(v, off);
       v = mx ? fmax(v, o) : v + o;
     }
@@ -300,6 +314,9 @@ __device__ __forceinline__ double axpby_elem(double sa, double a, double sb, dou
 }
 
 // Elementwise helpers.
+// d1 = s1 and d2 = s2 (n doubles each) and d3 = s3 (n3 doubles) in one launch.
+hipError_t launch_copy3(double* d1, const double* s1, double* d2, const double* s2, int64_t n,
+                        double* d3, const double* s3, int64_t n3, hipStream_t s);
 hipError_t launch_axpby(double* out, const double* a, double sa, const double* b,
                         double sb, int64_t n, hipStream_t s);  // out = sa*a + sb*b
 hipError_t launch_log(double* out, const double* a, int64_t n, hipStream_t s);

@@ -10,7 +10,7 @@ namespace {
 
 constexpr int kSub = 8;  // must match the pair kernel's sub-tile
 
-__global__ __launch_bounds__(1024) void reduce_kernel(const ReduceSpec spec, double* out) {
+__global__ __launch_bounds__(kReduceThreads) void reduce_kernel(const ReduceSpec spec, double* out) {
   reduce_block(spec, out);
 }
 
@@ -18,6 +18,15 @@ __global__ void axpby_kernel(double* out, const double* a, double sa,
                              const double* b, double sb, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = axpby_elem(sa, a[i], sb, b[i]);
+}
+__global__ void copy3_kernel(double* d1, const double* s1, double* d2, const double* s2,
+                             int64_t n, double* d3, const double* s3, int64_t n3) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    d1[i] = s1[i];
+    d2[i] = s2[i];
+  }
+  if (i < n3) d3[i] = s3[i];
 }
 __global__ void log_kernel(double* out, const double* a, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -170,7 +179,7 @@ LaunchCfg choose_mfma_launch(int64_t nt, int64_t ns, int num_cu, int force_split
 }
 
 hipError_t launch_reduce(const ReduceSpec& spec, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, s, spec, out);
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kReduceThreads), 0, s, spec, out);
   return hipGetLastError();
 }
 
@@ -178,6 +187,14 @@ hipError_t launch_axpby(double* out, const double* a, double sa, const double* b
                         double sb, int64_t n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(axpby_kernel, dim3(lin_blocks(n)), dim3(256), 0, s, out, a, sa, b, sb, n);
+  return hipGetLastError();
+}
+hipError_t launch_copy3(double* d1, const double* s1, double* d2, const double* s2, int64_t n,
+                        double* d3, const double* s3, int64_t n3, hipStream_t s) {
+  const int64_t len = n > n3 ? n : n3;
+  if (len <= 0) return hipSuccess;
+  hipLaunchKernelGGL(copy3_kernel, dim3(lin_blocks(len)), dim3(256), 0, s, d1, s1, d2, s2, n, d3,
+                     s3, n3);
   return hipGetLastError();
 }
 hipError_t launch_log(double* out, const double* a, int64_t n, hipStream_t s) {

@@ -93,7 +93,10 @@ typedef struct {
   int64_t iterations;  /* completed iterations (Sinkhorn: num_iter) */
   double objective;    /* transport cost <P, C> at exit */
   double residual;     /* L1 marginal infeasibility at exit */
-  double primal, dual; /* last logged primal / dual values (NaN if none) */
+  double primal, dual; /* primal / dual at the last check (NaN if none; the
+                        * single-instance LAMP solver skips the dual pass,
+                        * leaving dual NaN, when it stops on infeasibility
+                        * without logging) */
   double winf;         /* W_inf = max_ij C_ij */
   double eta_t;        /* LAMP: final eta_t; Sinkhorn: eta */
   double elapsed_s;    /* wall time of the solve */
