@@ -188,10 +188,10 @@ enum Layout : int { LAYOUT_AUTO = 0, LAYOUT_SPLIT = 1, LAYOUT_WIDE = 2 };
 constexpr int kThreads = 256;
 constexpr int kTile = 256;      // sources staged in LDS per step
 constexpr int kLanesPerTarget = 4;  // point kernel: adjacent lanes sharing a target
-constexpr int kMergeLanes = 16;     // split merge: adjacent lanes sharing a target
+constexpr int kMergeLanes = 4;      // split merge: adjacent lanes sharing a target
 // Workgroup-wide split layout (pair_pass_wide): one wave per source split,
-// one wave's worth of targets per workgroup.
-constexpr int kWideWaves = kMergeLanes;
+// one wave's worth of targets per workgroup, merged by one wave.
+constexpr int kWideWaves = 16;
 constexpr int kWideThreads = 64 * kWideWaves;
 constexpr int kWideTargets = 64 / kLanesPerTarget;
 constexpr int kMaxFields = 5;   // partial fields per target (BARY)
@@ -339,7 +339,12 @@ __device__ __forceinline__ double theta_update(double res, double c,
                                                double winf) {
   double dif = 2.0 * winf * (res - c) / etamu;
   double mx = fmax(dif, -dif);
-  double e = exp(dif - mx) * pow((th0 + 1.0) / (1.0 - th0), 1.0 - eta_mu);
+  double base = (th0 + 1.0) / (1.0 - th0);
+  if (eta_mu != 0.0) {
+    asm volatile("" ::: "memory");  // (keeps pow behind the branch: x^1 = x)
+    base = pow(base, 1.0 - eta_mu);
+  }
+  double e = exp(dif - mx) * base;
   double em = exp(-mx);
   double t = (e - em) / (e + em);
   if (adjust) t = fmin(fmax(t, minv), maxv);

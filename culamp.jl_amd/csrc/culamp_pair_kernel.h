@@ -261,7 +261,11 @@ __device__ __forceinline__ void epilogue_from(const PassDesc& d, const int64_t t
     if ((double)st.m <= (double)empty_max<PREC, R>() || st.S == 0.0) lse = -INFINITY;
     else if (PREC == 1) lse = ((double)st.m + log2(st.S) - st.qd) * kLn2;
     else lse = (double)st.m + log(st.S);
-    const double v = d.epi == EPI_SK_HALF ? log(E.mg) - lse : lse;
+    double v = lse;
+    if (d.epi == EPI_SK_HALF) {
+      asm volatile("" ::: "memory");  // (keeps the logarithm behind the branch)
+      v = log(E.mg) - lse;
+    }
     d.out0[t] = v;
     if (PREC == 1 && d.rec_out != nullptr) emit_record(d, t, v, E);
   } else {  // KIND_SUM
@@ -348,7 +352,14 @@ __device__ __forceinline__ double offset_from(const PassDesc& d, const TgtLoad& 
   double q = 0.0;
   if (KIND == KIND_SUM && d.tgtA != nullptr) q = d.gamma * L.ta;
   if (PREC == 1 && KIND == KIND_LSE && d.prev_offset) {
-    const double lse_prev = d.epi == EPI_SK_HALF ? log(L.mg) - L.prev : L.prev;
+    // (a branch on the uniform epilogue kind, kept a branch by the empty
+    // volatile statement, so that the other LSE passes do not evaluate the
+    // logarithm)
+    double lse_prev = L.prev;
+    if (d.epi == EPI_SK_HALF) {
+      asm volatile("" ::: "memory");
+      lse_prev = log(L.mg) - L.prev;
+    }
     q = isfinite(lse_prev) ? -lse_prev : 0.0;
   }
   return q;
@@ -360,9 +371,10 @@ __device__ __forceinline__ double target_offset(const PassDesc& d, int64_t t) {
 }
 
 // Combine the split partials of each target, then apply the epilogue.
-// kMergeLanes adjacent lanes share a target and read interleaved splits (one
-// split each up to kMergeLanes splits, so the loads are one round trip); they
-// combine by butterfly, so the result is deterministic.
+// kMergeLanes adjacent lanes share a target; lane `sub` folds splits sub,
+// sub + kMergeLanes, ... in order (unrolled, so that their loads go out
+// together), then the lanes combine by butterfly: a fixed order, so the
+// result is deterministic and equal to the workgroup-wide layout's.
 
 template <int PREC, int KIND>
 __device__ __forceinline__ void merge_body(const PassDesc& d, const Work& w, const int64_t block,
@@ -387,6 +399,7 @@ __device__ __forceinline__ void merge_body(const PassDesc& d, const Work& w, con
   st.S = st.Sc = st.Sy0 = st.Sy1 = st.Sy2 = 0.0;
   st.acc = KIND == KIND_MIN ? (R)INFINITY : (R)-INFINITY;
   if (ok) {
+#pragma unroll 4
     for (int q = sub; q < splits; q += kMergeLanes) {
       TState<R> p = st;
       if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
@@ -853,31 +866,33 @@ __global__ __launch_bounds__(kWideThreads) void pair_pass_wide(const PassDesc di
     for (int q = 0; q < NF; ++q) s_part[q][wave][tp] = f[q];
   }
   __syncthreads();
-  // merge_body over the LDS partials: kMergeLanes lanes per target, lane
-  // `sub` holding split `sub`
+  // merge_body over the LDS partials, by the first wave: kMergeLanes lanes
+  // per target, lane `sub` folding splits sub, sub + kMergeLanes, ... in order
   if (threadIdx.x >= kWideTargets * kMergeLanes) return;
   const bool ok = t < d.nt;
   TState<R> acc;
   acc.m = empty_max<PREC, R>();
   acc.S = acc.Sc = acc.Sy0 = acc.Sy1 = acc.Sy2 = 0.0;
   acc.acc = KIND == KIND_MIN ? (R)INFINITY : (R)-INFINITY;
-  if (ok && sub < nsplit) {
-    TState<R> p = acc;
-    if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
-      p.acc = (R)s_part[0][sub][tl];
-    } else if constexpr (KIND == KIND_SUM) {
-      p.S = s_part[0][sub][tl];
-      p.Sc = s_part[1][sub][tl];
-    } else {
-      p.m = (R)s_part[0][sub][tl];
-      p.S = s_part[1][sub][tl];
-      if (NF == 5) {
-        p.Sy0 = s_part[2][sub][tl];
-        p.Sy1 = s_part[3][sub][tl];
-        p.Sy2 = s_part[4][sub][tl];
+  if (ok) {
+    for (int q = sub; q < nsplit; q += kMergeLanes) {
+      TState<R> p = acc;
+      if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
+        p.acc = (R)s_part[0][q][tl];
+      } else if constexpr (KIND == KIND_SUM) {
+        p.S = s_part[0][q][tl];
+        p.Sc = s_part[1][q][tl];
+      } else {
+        p.m = (R)s_part[0][q][tl];
+        p.S = s_part[1][q][tl];
+        if (NF == 5) {
+          p.Sy0 = s_part[2][q][tl];
+          p.Sy1 = s_part[3][q][tl];
+          p.Sy2 = s_part[4][q][tl];
+        }
       }
+      combine_state<PREC, KIND, R>(acc, p);
     }
-    combine_state<PREC, KIND, R>(acc, p);
   }
   combine_lanes<PREC, KIND, kMergeLanes, R>(acc);
   if (!own) return;
