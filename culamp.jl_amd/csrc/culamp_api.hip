@@ -424,13 +424,30 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
     return run_pass(ctx, d3, a->precision, KIND_SUM);
   };
   if (a->verbose) emit(log_fn, user, "time(s),iter,infeas,ot_objective,dual\n");
+  // A check's sums are read back after the next iteration's half steps are
+  // enqueued, so the GPU runs them while the host waits; phi and psi are
+  // kept at the check, and a stop (or the time limit right after) restores
+  // them, so the returned state is the check's.
+  double *phi_keep, *psi_keep;
+  TRY(RB.alloc(&phi_keep, n)); TRY(RB.alloc(&psi_keep, m));
+  auto restore = [&]() {
+    recy_ok = false;  // rec_y describes the undone half step
+    return launch_copy3(st.phi, phi_keep, nullptr, nullptr, n, st.psi, psi_keep, m, ctx->stream);
+  };
   int64_t num_iter = 0;
   int converged = 0;
+  bool ahead = false;  // this iteration's half steps were enqueued at the last check
   for (int64_t it = 1; it <= a->itermax; ++it) {
     const double elapsed = now_s() - t0;
-    if (elapsed > a->tmax) break;
-    TRY(half_rows());
-    TRY(half_cols());
+    if (elapsed > a->tmax) {
+      if (ahead) HIP_TRY(restore());
+      break;
+    }
+    if (!ahead) {
+      TRY(half_rows());
+      TRY(half_cols());
+    }
+    ahead = false;
     if ((it - 1) % frequency == 0) {
       TRY(resid());
       ReduceSpec rs{};
@@ -439,8 +456,15 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
       rs.op[1] = 0; rs.a[1] = st.cost; rs.len[1] = n;
       rs.op[2] = 2; rs.a[2] = st.psi; rs.b[2] = pb->c; rs.len[2] = m;
       rs.op[3] = 2; rs.a[3] = st.phi; rs.b[3] = pb->r; rs.len[3] = n;
+      TRY(reduce_start(ctx, rs));
+      const bool spec = it < a->itermax;
+      if (spec) {
+        HIP_TRY(launch_copy3(phi_keep, st.phi, nullptr, nullptr, n, psi_keep, st.psi, m, ctx->stream));
+        TRY(half_rows());
+        TRY(half_cols());
+      }
       double v[4];
-      TRY(reduce_to_host(ctx, rs, v));
+      TRY(reduce_finish(ctx, 4, v));
       const double rr = v[0];
       if (a->verbose) {
         emit(log_fn, user, fmt("%.6e", elapsed) + "," + std::to_string(it) + "," +
@@ -449,8 +473,10 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
       }
       if (rr <= a->epsilon / 6.0) {
         converged = 1;
+        if (spec) HIP_TRY(restore());
         break;
       }
+      ahead = spec;
     }
     num_iter += 1;
   }

@@ -314,7 +314,8 @@ __device__ __forceinline__ double axpby_elem(double sa, double a, double sb, dou
 }
 
 // Elementwise helpers.
-// d1 = s1 and d2 = s2 (n doubles each) and d3 = s3 (n3 doubles) in one launch.
+// d1 = s1 and d2 = s2 (n doubles each; d2 may be null) and d3 = s3 (n3
+// doubles) in one launch.
 hipError_t launch_copy3(double* d1, const double* s1, double* d2, const double* s2, int64_t n,
                         double* d3, const double* s3, int64_t n3, hipStream_t s);
 hipError_t launch_axpby(double* out, const double* a, double sa, const double* b,

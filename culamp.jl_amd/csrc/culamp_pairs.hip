@@ -24,7 +24,7 @@ __global__ void copy3_kernel(double* d1, const double* s1, double* d2, const dou
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     d1[i] = s1[i];
-    d2[i] = s2[i];
+    if (d2 != nullptr) d2[i] = s2[i];
   }
   if (i < n3) d3[i] = s3[i];
 }
