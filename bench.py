@@ -317,6 +317,7 @@ def bench_clouds(ctx, a, log) -> dict:
             gbs = (info["pairs"] * b_pair + info["passes"] * pass_bytes(n, m, 0)) / 1e9 / dt
             rec[prec] = {"seconds": dt, "iterations": info["iterations"],
                          "iterations_per_s": info["iterations"] / dt,
+                         "pairs_per_s": info["pairs"] / dt,
                          "residual": info["residual"], "objective": info["objective"],
                          "converged": info["converged"], "dense_eq_GBps": gbs}
         rec["speedup_mixed_vs_fp64"] = rec["fp64"]["seconds"] / rec["mixed"]["seconds"]
@@ -541,6 +542,16 @@ def main() -> None:
         log(f"fp64 plain: {d64:.3f}s, {i64['iterations']} it, infeas {i64['residual']:.3e}")
 
     roofline = lse_roofline(ctx, x, y, nu, a, log)
+    # the compute side: one v_exp_f32 per pair, against the device's measured
+    # exponential throughput (independent-chain micro-kernel)
+    rates = ctx.probe_rates()
+    roofline["compute"] = {
+        "bound": "v_exp_f32 + VALU", "unit": "exp/s",
+        "achieved": roofline["pairs_per_s"], "peak_measured": rates["exp_per_s"],
+        "frac": roofline["pairs_per_s"] / rates["exp_per_s"],
+        "fma_per_s_measured": rates["fma_per_s"]}
+    log(f"probe: {rates['exp_per_s'] / 1e12:.2f} T exp/s, {rates['fma_per_s'] / 1e12:.2f} T fma/s; "
+        f"row-LSE pass {roofline['pairs_per_s'] / 1e12:.2f} T pairs/s")
 
     cpu = None
     if not a.no_cpu:

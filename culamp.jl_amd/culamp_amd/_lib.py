@@ -99,6 +99,7 @@ SIGNATURES = {
     "culamp_ctx_set_launch": (_I, [_P, _I, _I]),
     "culamp_ctx_set_layout": (_I, [_P, _I]),
     "culamp_ctx_set_source_records": (_I, [_P, _I]),
+    "culamp_probe_rates": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "culamp_malloc": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
     "culamp_free": (_I, [_P, _P]),
     "culamp_memcpy_htod": (_I, [_P, _P, _P, C.c_size_t]),
@@ -261,6 +262,13 @@ class Context:
         records from each pass's epilogue to the next pass (identical results
         either way)."""
         check(self.lib.culamp_ctx_set_source_records(self.handle, 1 if enable else 0))
+
+    def probe_rates(self) -> dict:
+        """Measured FP32 exponential and fused multiply-add throughput of the
+        device, lane operations per second."""
+        e, f = C.c_double(), C.c_double()
+        check(self.lib.culamp_probe_rates(self.handle, C.byref(e), C.byref(f)))
+        return {"exp_per_s": e.value, "fma_per_s": f.value}
 
     def capture(self, fn) -> "Graph":
         """Records the kernel-level calls made by fn() into a replayable graph."""

@@ -136,6 +136,11 @@ int culamp_ctx_set_layout(culamp_ctx *ctx, int layout);
  * weight) of the pass after it, which then skips converting its sources; 0
  * makes every pass convert its own.  Both give identical results. */
 int culamp_ctx_set_source_records(culamp_ctx *ctx, int enable);
+/* Compute-rate probe (no reference counterpart; measurement): the device's
+ * throughput of the FP32 exponential (v_exp_f32, one per pair in every pass)
+ * and of the FP32 fused multiply-add, in lane operations per second, from
+ * independent-chain micro-kernels timed on the context's stream (best of 3). */
+int culamp_probe_rates(culamp_ctx *ctx, double *exp_per_s, double *fma_per_s);
 /* Device memory helpers (stream-ordered on the context's stream). */
 int culamp_malloc(culamp_ctx *ctx, size_t bytes, void **dptr);
 int culamp_free(culamp_ctx *ctx, void *dptr);

@@ -314,3 +314,12 @@ def test_edge_cases(gpu_ctx, oracle):
         ref = oracle.lse_rows(x, y, big.get(), 0.01, 1.0, 2.0, 2.0)
         assert np.all(np.isfinite(o.get()))
         assert np.max(np.abs(o.get() - ref) / np.abs(ref)) < (1e-12 if prec == FP64 else 1e-6)
+
+
+def test_probe_rates(gpu_ctx):
+    """The compute-rate probe returns finite, plausible device rates (lane
+    operations per second): exponentials at most as fast as FMAs."""
+    r = gpu_ctx.probe_rates()
+    say("probe", r)
+    assert 1e11 < r["exp_per_s"] < 1e15 and 1e11 < r["fma_per_s"] < 1e15
+    assert r["exp_per_s"] <= 1.05 * r["fma_per_s"]
