@@ -631,7 +631,7 @@ __device__ __forceinline__ void group_sync() {
 // t0 + r * (NTHR / kLanesPerTarget) + lane / kLanesPerTarget.  On return the
 // lanes have combined their states; the owner lane of each target keeps its
 // index in tidx, the others hold `none`.
-template <int PREC, int KIND, int COST, int RPT, int NTHR, int TILE, typename R>
+template <int PREC, int KIND, int COST, int RPT, int NTHR, int TILE, typename R, bool REC = false>
 __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* s_w,
                                            const int tid, const int64_t t0, const int64_t s0,
                                            const int64_t s1, const int64_t none,
@@ -653,7 +653,8 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
   constexpr bool kScaled = kLazy && COST == COST_SQL2;
   const double csc = kScaled ? lse_coord_scale(d.k) : 1.0;
   // staged records from the previous pass's epilogue (mixed LSE / SUM)
-  const bool use_rec = PREC == 1 && kExpKind && d.rec != nullptr;
+  // (REC: this instantiation reads them; the launcher picks it when d.rec is set)
+  constexpr bool use_rec = REC && PREC == 1 && kExpKind;
   // This thread's source of the next tile, held in registers from its load
   // until it is written to LDS: the loads of a tile go out together and one
   // tile ahead of the pair loop (every index is clamped into range, so no
@@ -665,7 +666,7 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
   auto fetch = [&](const int64_t base) {
     const int64_t last = s1 - 1;
     const int64_t sidx = min(base + tid, last);
-    if (use_rec) {
+    if constexpr (use_rec) {
       rq = d.rec[sidx];
       rqw = d.recw[sidx];
       return;
@@ -721,7 +722,7 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
     if (tid < cnt) {
       SrcPt<R> sp;
       R wsrc;
-      if (use_rec) {
+      if constexpr (use_rec) {
         sp.x = (R)rq.x;
         sp.y = (R)rq.y;
         sp.z = (R)rq.z;
@@ -800,7 +801,7 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
   }
 }
 
-template <int PREC, int KIND, int COST, int RPT>
+template <int PREC, int KIND, int COST, int RPT, bool REC = false>
 __device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, const int64_t tb,
                                           const int split, const int splits,
                                           const int64_t split_len, const int64_t ntp) {
@@ -812,8 +813,8 @@ __device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, cons
   const int64_t s1 = min(d.ns, s0 + split_len);
   int64_t tidx[RPT];
   TState<R> st[RPT];
-  pair_accum<PREC, KIND, COST, RPT, kThreads, kTile, R>(d, s_p, s_w, threadIdx.x, tb * TPB * RPT,
-                                                         s0, s1, ntp, tidx, st);
+  pair_accum<PREC, KIND, COST, RPT, kThreads, kTile, R, REC>(d, s_p, s_w, threadIdx.x,
+                                                              tb * TPB * RPT, s0, s1, ntp, tidx, st);
   finish_pass<PREC, KIND, RPT, R>(d, w, splits, split, ntp, tidx, st);
 }
 
@@ -824,7 +825,7 @@ __device__ __forceinline__ void pair_body(const PassDesc& d, const Work& w, cons
 // stages its own split's sources.  The per-split states and their merge are
 // the split path's (pair_body with the same split_len, then merge_body), so
 // both give identical results.
-template <int PREC, int KIND, int COST>
+template <int PREC, int KIND, int COST, bool REC = false>
 __global__ __launch_bounds__(kWideThreads) void pair_pass_wide(const PassDesc din,
                                                                const int64_t split_len) {
   using R = typename std::conditional<PREC == 0, double, float>::type;
@@ -854,8 +855,8 @@ __global__ __launch_bounds__(kWideThreads) void pair_pass_wide(const PassDesc di
   }
   int64_t tidx[1];
   TState<R> st[1];
-  pair_accum<PREC, KIND, COST, 1, 64, TILE, R>(d, s_p[wave], s_w[wave], lane, t0, s0, s1, none,
-                                                tidx, st);
+  pair_accum<PREC, KIND, COST, 1, 64, TILE, R, REC>(d, s_p[wave], s_w[wave], lane, t0, s0, s1,
+                                                     none, tidx, st);
   if (tidx[0] != none) {
     const int tp = (int)(tidx[0] - t0);
     double f[5];
@@ -901,13 +902,13 @@ __global__ __launch_bounds__(kWideThreads) void pair_pass_wide(const PassDesc di
   epilogue_from<PREC, KIND, R>(d, t, acc, E);
 }
 
-template <int PREC, int KIND, int COST, int RPT>
+template <int PREC, int KIND, int COST, int RPT, bool REC = false>
 __global__ __launch_bounds__(kThreads) void pair_pass(const PassDesc din, const Work w,
                                                       const int splits,
                                                       const int64_t split_len) {
   const PassDesc d = resolve_desc(din);
-  pair_body<PREC, KIND, COST, RPT>(d, w, blockIdx.x, blockIdx.y, splits, split_len,
-                                   (int64_t)gridDim.x * (kThreads / kLanesPerTarget) * RPT);
+  pair_body<PREC, KIND, COST, RPT, REC>(d, w, blockIdx.x, blockIdx.y, splits, split_len,
+                                        (int64_t)gridDim.x * (kThreads / kLanesPerTarget) * RPT);
 }
 
 template <int PREC, int KIND, int COST, int RPT>
@@ -1441,25 +1442,34 @@ hipError_t launch_mfma_ks(const PassDesc& d, const LaunchCfg& c, const Work& w, 
   return launch_merge<1, KIND>(d, c, w, c.tblocks * 128, s);
 }
 
-template <int PREC, int KIND, int COST>
-hipError_t launch_rpt(const PassDesc& d, const LaunchCfg& c, const Work& w, hipStream_t s) {
+template <int PREC, int KIND, int COST, bool REC>
+hipError_t launch_rpt_rec(const PassDesc& d, const LaunchCfg& c, const Work& w, hipStream_t s) {
   if constexpr (KIND != KIND_BARY) {  // (five FP64 fields do not fit the wide layout's registers)
     if (c.wide) {
-      hipLaunchKernelGGL((pair_pass_wide<PREC, KIND, COST>), dim3((unsigned)c.tblocks),
+      hipLaunchKernelGGL((pair_pass_wide<PREC, KIND, COST, REC>), dim3((unsigned)c.tblocks),
                          dim3(kWideThreads), 0, s, d, c.split_len);
       return hipGetLastError();
     }
   }
   dim3 grid((unsigned)c.tblocks, (unsigned)c.splits);
   switch (c.rpt) {
-    case 1: hipLaunchKernelGGL((pair_pass<PREC, KIND, COST, 1>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;
-    case 2: hipLaunchKernelGGL((pair_pass<PREC, KIND, COST, 2>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;
-    case 4: hipLaunchKernelGGL((pair_pass<PREC, KIND, COST, 4>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;
+    case 1: hipLaunchKernelGGL((pair_pass<PREC, KIND, COST, 1, REC>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;
+    case 2: hipLaunchKernelGGL((pair_pass<PREC, KIND, COST, 2, REC>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;
+    case 4: hipLaunchKernelGGL((pair_pass<PREC, KIND, COST, 4, REC>), grid, dim3(kThreads), 0, s, d, w, c.splits, c.split_len); break;
     default: return hipErrorInvalidValue;
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return launch_merge<PREC, KIND>(d, c, w, c.tblocks * (kThreads / kLanesPerTarget) * c.rpt, s);
+}
+
+// Passes whose sources come as staged records (mixed LSE / SUM with d.rec set)
+// run an instantiation that reads only records; all others convert sources.
+template <int PREC, int KIND, int COST>
+hipError_t launch_rpt(const PassDesc& d, const LaunchCfg& c, const Work& w, hipStream_t s) {
+  if constexpr (PREC == 1 && (KIND == KIND_LSE || KIND == KIND_SUM))
+    if (d.rec != nullptr) return launch_rpt_rec<PREC, KIND, COST, true>(d, c, w, s);
+  return launch_rpt_rec<PREC, KIND, COST, false>(d, c, w, s);
 }
 
 template <int PREC, int KIND>
