@@ -663,9 +663,11 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
   double px = 0.0, py = 0.0, pz = 0.0, pa = 0.0, pb = 1.0;
   float4 rq = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float rqw = 0.0f;
-  auto fetch = [&](const int64_t base) {
-    const int64_t last = s1 - 1;
-    const int64_t sidx = min(base + tid, last);
+  // (tile offsets are 32-bit, relative to s0: a split holds fewer than 2^31
+  // sources)
+  const int span = s1 > s0 ? (int)(s1 - s0) : 0;
+  auto fetch = [&](const int off) {
+    const int64_t sidx = s0 + min(off + tid, span - 1);
     if constexpr (use_rec) {
       rq = d.rec[sidx];
       rqw = d.recw[sidx];
@@ -677,7 +679,7 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
     pa = d.srcA != nullptr ? d.srcA[sidx] : 0.0;
     pb = d.srcB != nullptr ? d.srcB[sidx] : 1.0;
   };
-  if (s0 < s1) fetch(s0);
+  if (span > 0) fetch(0);
 
   R tx[RPT], ty[RPT], tz[RPT], tq[RPT], tqm[RPT];
   double dl[RPT];
@@ -715,32 +717,37 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
     dl[r] = 0.0;
   }
 
-  for (int64_t base = s0; base < s1; base += TILE) {
-    const int cnt = (int)min((int64_t)TILE, s1 - base);
+  auto stage = [&]() {
+    SrcPt<R> sp;
+    R wsrc;
+    if constexpr (use_rec) {
+      sp.x = (R)rq.x;
+      sp.y = (R)rq.y;
+      sp.z = (R)rq.z;
+      sp.a = (R)rq.w;
+      wsrc = (R)rqw;
+    } else {
+      sp.x = (R)(kScaled ? px * csc : px);
+      sp.y = (R)(kScaled ? py * csc : py);
+      sp.z = (R)(kScaled ? pz * csc : pz);
+      source_terms_v<PREC, kExpKind>(d, pa, pb, &sp.a, &wsrc);
+    }
+    s_p[tid] = sp;
+    s_w[tid] = wsrc;
+  };
+  for (int off = 0; off < span; off += TILE) {
+    const int cnt = min(TILE, span - off);
     const int cntp = (cnt + kSub * G - 1) / (kSub * G) * (kSub * G);
     group_sync<NTHR>();
-    if (tid < cnt) {
-      SrcPt<R> sp;
-      R wsrc;
-      if constexpr (use_rec) {
-        sp.x = (R)rq.x;
-        sp.y = (R)rq.y;
-        sp.z = (R)rq.z;
-        sp.a = (R)rq.w;
-        wsrc = (R)rqw;
-      } else {
-        sp.x = (R)(kScaled ? px * csc : px);
-        sp.y = (R)(kScaled ? py * csc : py);
-        sp.z = (R)(kScaled ? pz * csc : pz);
-        source_terms_v<PREC, kExpKind>(d, pa, pb, &sp.a, &wsrc);
-      }
-      s_p[tid] = sp;
-      s_w[tid] = wsrc;
-    } else if (tid < cntp) {
+    // every thread stages its source (past the split's end, a copy of the
+    // last one); in a partial tile the slots from cnt to the sub-tile
+    // boundary cntp are then overwritten by padding that adds nothing
+    stage();
+    if (tid >= cnt && tid < cntp) {
       s_p[tid] = SrcPt<R>{(R)0, (R)0, (R)0, (R)-INFINITY};
       s_w[tid] = (R)0;
     }
-    if (base + TILE < s1) fetch(base + TILE);
+    if (off + TILE < span) fetch(off + TILE);
     group_sync<NTHR>();
 
     if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
