@@ -717,7 +717,17 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
     dl[r] = 0.0;
   }
 
-  auto stage = [&]() {
+  // Planar tiles (squared Euclidean): when every staged source of a tile and
+  // every target of the group has z == 0, the z terms of the cost are exact
+  // zeros (fma(-0, 0, a) == a, fma(g, g, 0 * 0) == g * g), so the tile runs a
+  // loop without them and gives bit-identical results.
+  constexpr bool kPlanarOk = COST == COST_SQL2 && kExpKind;
+  bool tgt_nz = false;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) tgt_nz = tgt_nz || tz[r] != (R)0;
+  __shared__ int s_nz[NTHR / 64];
+
+  auto stage = [&]() -> bool {
     SrcPt<R> sp;
     R wsrc;
     if constexpr (use_rec) {
@@ -734,6 +744,7 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
     }
     s_p[tid] = sp;
     s_w[tid] = wsrc;
+    return sp.z != (R)0;
   };
   for (int off = 0; off < span; off += TILE) {
     const int cnt = min(TILE, span - off);
@@ -742,13 +753,30 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
     // every thread stages its source (past the split's end, a copy of the
     // last one); in a partial tile the slots from cnt to the sub-tile
     // boundary cntp are then overwritten by padding that adds nothing
-    stage();
+    const bool src_nz = stage();
     if (tid >= cnt && tid < cntp) {
       s_p[tid] = SrcPt<R>{(R)0, (R)0, (R)0, (R)-INFINITY};
       s_w[tid] = (R)0;
     }
     if (off + TILE < span) fetch(off + TILE);
+    bool planar = false;
+    if constexpr (kPlanarOk) {
+      // any nonzero z in the wave, then (workgroup groups) across its waves
+      // through one LDS word per wave, read after the tile's barrier
+      const bool wave_nz = __builtin_amdgcn_ballot_w64(src_nz || tgt_nz) != 0;
+      if constexpr (NTHR == 64) {
+        planar = !wave_nz;
+      } else {
+        s_nz[tid / 64] = wave_nz ? 1 : 0;
+      }
+    }
     group_sync<NTHR>();
+    if constexpr (kPlanarOk && NTHR != 64) {
+      int any = 0;
+#pragma unroll
+      for (int q = 0; q < NTHR / 64; ++q) any |= s_nz[q];
+      planar = __builtin_amdgcn_readfirstlane(any) == 0;
+    }
 
     if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
       for (int j = g; j < cnt; j += G) {
@@ -767,36 +795,52 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
     } else {
       // lane g takes sources j0 + u*G + g: the G lanes of a target read G
       // consecutive LDS words per step
-      for (int j0 = 0; j0 < cntp; j0 += kSub * G) {
-        R v[RPT][kSub];
-        R Cv[RPT][kSub];
-        R wv[kSub], px[kSub], py[kSub], pz[kSub];
+      auto tile_pairs = [&](auto planar_tag) {
+        constexpr bool kPlanar = decltype(planar_tag)::value;
+        for (int j0 = 0; j0 < cntp; j0 += kSub * G) {
+          R v[RPT][kSub];
+          R Cv[RPT][kSub];
+          R wv[kSub], px[kSub], py[kSub], pz[kSub];
 #pragma unroll
-        for (int u = 0; u < kSub; ++u) {
-          const int j = j0 + u * G + g;
-          const SrcPt<R> sp = s_p[j];
-          const R sx = sp.x, sy = sp.y, sz = sp.z;
-          const R sa = sp.a;
-          wv[u] = s_w[j];
-          px[u] = sx; py[u] = sy; pz[u] = sz;
+          for (int u = 0; u < kSub; ++u) {
+            const int j = j0 + u * G + g;
+            const SrcPt<R> sp = s_p[j];
+            const R sx = sp.x, sy = sp.y, sz = sp.z;
+            const R sa = sp.a;
+            wv[u] = s_w[j];
+            px[u] = sx; py[u] = sy; pz[u] = sz;
 #pragma unroll
-          for (int r = 0; r < RPT; ++r) {
-            if constexpr (kScaled) {
-              const R dr = tx[r] - sx, dg = ty[r] - sy, db = tz[r] - sz;
-              Cv[r][u] = (R)0;
-              v[r][u] = fma(-dr, dr, fma(-dg, dg, fma(-db, db, sa + tqm[r])));
-            } else {
-              const R C = pair_cost<COST>(tx[r], ty[r], tz[r], sx, sy, sz, pexp);
-              Cv[r][u] = C;
-              v[r][u] = fma(k, C, sa + (kLazy ? tqm[r] : tq[r]));
+            for (int r = 0; r < RPT; ++r) {
+              const R dr = tx[r] - sx, dg = ty[r] - sy;
+              if constexpr (kScaled) {
+                Cv[r][u] = (R)0;
+                const R a0 = sa + tqm[r];
+                if constexpr (kPlanar) {
+                  v[r][u] = fma(-dr, dr, fma(-dg, dg, a0));
+                } else {
+                  const R db = tz[r] - sz;
+                  v[r][u] = fma(-dr, dr, fma(-dg, dg, fma(-db, db, a0)));
+                }
+              } else {
+                const R C = kPlanar ? fma(dr, dr, dg * dg)
+                                    : pair_cost<COST>(tx[r], ty[r], tz[r], sx, sy, sz, pexp);
+                Cv[r][u] = C;
+                v[r][u] = fma(k, C, sa + (kLazy ? tqm[r] : tq[r]));
+              }
             }
           }
-        }
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-          if constexpr (kLazy) lse_lazy<kSub>(st[r], v[r], wv, tq[r], tqm[r], dl[r]);
-          else accumulate<PREC, KIND, kSub, R>(st[r], v[r], wv, Cv[r], px, py, pz);
+          for (int r = 0; r < RPT; ++r) {
+            if constexpr (kLazy) lse_lazy<kSub>(st[r], v[r], wv, tq[r], tqm[r], dl[r]);
+            else accumulate<PREC, KIND, kSub, R>(st[r], v[r], wv, Cv[r], px, py, pz);
+          }
         }
+      };
+      if constexpr (kPlanarOk) {
+        if (planar) tile_pairs(std::true_type{});
+        else tile_pairs(std::false_type{});
+      } else {
+        tile_pairs(std::false_type{});
       }
     }
   }

@@ -219,6 +219,58 @@ def test_wide_layout_matches_split_layout(gpu_ctx, oracle, n, m):
         gpu_ctx.set_layout("auto")
 
 
+@pytest.mark.parametrize("n,m", [(4096, 4096), (3000, 5000), (700, 333)])
+def test_planar_tiles_match_general_loop(gpu_ctx, n, m):
+    """Squared-Euclidean passes over points with z == 0 run a loop without the
+    z terms.  Giving some sources (or one target) a z so small that its square
+    vanishes in the pass's arithmetic (FP64: 1e-170; FP32: 1e-30) sends those
+    tiles through the general loop instead; every output must keep its bits
+    (wide and split layouts, FP64 and mixed, row LSE, column marginal and
+    cost, Sinkhorn half step and residual, barycentric x / y)."""
+    from culamp_amd import FP64, MIXED
+    from culamp_amd import kernels as K
+    say("planar tiles n", n, "m", m)
+    rng = np.random.default_rng(n + m)
+    x, y = rng.uniform(size=(n, 3)), rng.uniform(size=(m, 3))
+    x[:, 2] = 0.0
+    y[:, 2] = 0.0
+    th = rng.uniform(-1, 1, m)
+    r = rng.uniform(0.1, 1, n)
+    r /= r.sum()
+    lz = rng.uniform(-1, 2, n)
+    phi = rng.normal(0, 1, n)
+    dth, dr, dlz, dphi = dev(gpu_ctx, th), dev(gpu_ctx, r), dev(gpu_ctx, lz), dev(gpu_ctx, phi)
+    o, res, cost = out(gpu_ctx, n), out(gpu_ctx, m), out(gpu_ctx, m)
+    rres, rcost, b = out(gpu_ctx, n), out(gpu_ctx, n), out(gpu_ctx, 3 * n)
+
+    def run(xa, ya, prec):
+        dx, dy = dev(gpu_ctx, xa), dev(gpu_ctx, ya)
+        got = []
+        K.warp_logsumexp_spp_ct_fused(gpu_ctx, o, dx, dy, dth, 0.02, 1.0, 3.0, 2.0, prec)
+        got.append(o.get())
+        K.residual_spp_c(gpu_ctx, res, cost, dx, dy, dr, dth, dlz, 0.02, 1.0, 3.0, 2.0, prec)
+        got += [res.get(), cost.get()]
+        K.warp_logsumexp_ct_fused(gpu_ctx, o, dx, dy, dr, dth, 0.02, 3.0, 2.0, prec)
+        got.append(o.get())
+        K.residual_opt(gpu_ctx, rres, rcost, dx, dy, dr, dphi, dth, 0.02, 3.0, 2.0, prec)
+        got += [rres.get(), rcost.get()]
+        if prec == FP64:
+            K.naive_findmaxindex_ct(gpu_ctx, b, dx, dy, dphi, dth, 0.02, 3.0, 2.0)
+            got.append(b.get().reshape(-1, 3)[:, :2].copy())
+        return got
+
+    for prec, tiny in ((FP64, 1e-170), (MIXED, 1e-30)):
+        want = run(x, y, prec)
+        ys = y.copy()
+        ys[::97, 2] = tiny
+        xs = x.copy()
+        xs[min(5, n - 1), 2] = tiny
+        for label, (xa, ya) in (("sources", (x, ys)), ("target", (xs, y))):
+            got = run(xa, ya, prec)
+            for k, (w, g) in enumerate(zip(want, got)):
+                assert np.array_equal(w, g), (prec, label, k)
+
+
 def test_captured_graph_replays_match_eager(gpu_ctx, oracle):
     """Kernel-level calls captured into a graph: every replay equals the eager
     result, with new inputs uploaded and eager work run between the replays."""

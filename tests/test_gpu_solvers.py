@@ -204,6 +204,48 @@ def test_lamp_source_records_identical(gpu_ctx, layout, p, eta, stop):
         gpu_ctx.set_layout("auto")
 
 
+@pytest.mark.parametrize("n,m", [(4096, 4096), (700, 450)])
+def test_planar_solves_match_general_loop(gpu_ctx, n, m):
+    """Mixed LAMP and Sinkhorn on planar clouds (z == 0, whose squared-
+    Euclidean passes run the loop without z terms) against the same clouds
+    with z = 1e-30 on every 61st target point, a z whose square vanishes in
+    FP32 (and is absorbed by every FP64 cost of these distinct points) but
+    which sends the tiles holding it through the general loop: identical
+    logs and outputs."""
+    import culamp_amd
+    say("planar solves", n, m)
+    rng = np.random.default_rng(41)
+    x, y = rng.uniform(size=(n, 3)), rng.uniform(size=(m, 3))
+    x[:, 2] = 0.0
+    y[:, 2] = 0.0
+    yz = y.copy()
+    yz[::61, 2] = 1e-30
+    r, c = hist(42, n), hist(43, m)
+    lamp = culamp_amd.EOTArgs(eta_p=1e-2, epsilon=6e-6, B=1.1, itermax=20000, verbose=True,
+                              precision="mixed", stop="infeas")
+    sk = culamp_amd.EOTArgs(eta_p=2e-2, epsilon=1e-6, itermax=2000, verbose=True,
+                            precision="mixed", stop="infeas")
+    out = []
+    for ya in (y, yz):
+        pb = culamp_amd.Problem(x, ya, r, c, gpu_ctx)
+        try:
+            l1, l2 = [], []
+            a = pb.lamp(lamp, 10, 2.0, l1.append)
+            b = pb.sinkhorn(sk, 10, 2.0, log=l2.append)
+        finally:
+            pb.close()
+        out.append(([l.split(",", 1)[1] for l in l1[1:]], [l.split(",", 1)[1] for l in l2[1:]],
+                    a, b))
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    for k, nk in ((2, 5), (3, 4)):
+        for a, b in zip(out[0][k][:nk], out[1][k][:nk]):
+            if a is not None and a.ndim == 2 and a.shape[1] == 3:
+                # barycentric projections: z averages the points' own z
+                a, b = a[:, :2], b[:, :2]
+            assert np.array_equal(a, b)
+    assert out[0][2][5]["iterations"] == out[1][2][5]["iterations"]
+
+
 @pytest.mark.parametrize("p", [2.0, 1.0, math.inf])
 def test_sinkhorn_source_records_identical(gpu_ctx, p):
     """Mixed-precision Sinkhorn and annealed Sinkhorn with staged source
