@@ -727,25 +727,10 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
   for (int r = 0; r < RPT; ++r) tgt_nz = tgt_nz || tz[r] != (R)0;
   __shared__ int s_nz[NTHR / 64];
 
-  auto stage = [&]() -> bool {
-    SrcPt<R> sp;
-    R wsrc;
-    if constexpr (use_rec) {
-      sp.x = (R)rq.x;
-      sp.y = (R)rq.y;
-      sp.z = (R)rq.z;
-      sp.a = (R)rq.w;
-      wsrc = (R)rqw;
-    } else {
-      sp.x = (R)(kScaled ? px * csc : px);
-      sp.y = (R)(kScaled ? py * csc : py);
-      sp.z = (R)(kScaled ? pz * csc : pz);
-      source_terms_v<PREC, kExpKind>(d, pa, pb, &sp.a, &wsrc);
-    }
-    s_p[tid] = sp;
-    s_w[tid] = wsrc;
-    return sp.z != (R)0;
-  };
+  // One wave's tile (the workgroup-wide layout) knows whether it is planar
+  // before it writes LDS: a planar tile then carries each source's weight in
+  // the z slot, so the pair loop reads a source with one 16-byte load.
+  constexpr bool kPacked = kPlanarOk && NTHR == 64;
   for (int off = 0; off < span; off += TILE) {
     const int cnt = min(TILE, span - off);
     const int cntp = (cnt + kSub * G - 1) / (kSub * G) * (kSub * G);
@@ -753,25 +738,45 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
     // every thread stages its source (past the split's end, a copy of the
     // last one); in a partial tile the slots from cnt to the sub-tile
     // boundary cntp are then overwritten by padding that adds nothing
-    const bool src_nz = stage();
+    bool planar = false;
+    {
+      SrcPt<R> sp;
+      R wsrc;
+      if constexpr (use_rec) {
+        sp.x = (R)rq.x;
+        sp.y = (R)rq.y;
+        sp.z = (R)rq.z;
+        sp.a = (R)rq.w;
+        wsrc = (R)rqw;
+      } else {
+        sp.x = (R)(kScaled ? px * csc : px);
+        sp.y = (R)(kScaled ? py * csc : py);
+        sp.z = (R)(kScaled ? pz * csc : pz);
+        source_terms_v<PREC, kExpKind>(d, pa, pb, &sp.a, &wsrc);
+      }
+      if constexpr (kPlanarOk) {
+        // any nonzero z in the wave, then (workgroup groups) across its waves
+        // through one LDS word per wave, read after the tile's barrier
+        const bool wave_nz = __builtin_amdgcn_ballot_w64(sp.z != (R)0 || tgt_nz) != 0;
+        if constexpr (kPacked) {
+          planar = !wave_nz;
+          if (planar) sp.z = wsrc;
+        } else {
+          s_nz[tid / 64] = wave_nz ? 1 : 0;
+        }
+      }
+      s_p[tid] = sp;
+      if constexpr (!kPacked) s_w[tid] = wsrc;
+      else if (!planar) s_w[tid] = wsrc;
+    }
     if (tid >= cnt && tid < cntp) {
+      // (packed or not, the padding's weight is 0 and its z is 0)
       s_p[tid] = SrcPt<R>{(R)0, (R)0, (R)0, (R)-INFINITY};
       s_w[tid] = (R)0;
     }
     if (off + TILE < span) fetch(off + TILE);
-    bool planar = false;
-    if constexpr (kPlanarOk) {
-      // any nonzero z in the wave, then (workgroup groups) across its waves
-      // through one LDS word per wave, read after the tile's barrier
-      const bool wave_nz = __builtin_amdgcn_ballot_w64(src_nz || tgt_nz) != 0;
-      if constexpr (NTHR == 64) {
-        planar = !wave_nz;
-      } else {
-        s_nz[tid / 64] = wave_nz ? 1 : 0;
-      }
-    }
     group_sync<NTHR>();
-    if constexpr (kPlanarOk && NTHR != 64) {
+    if constexpr (kPlanarOk && !kPacked) {
       int any = 0;
 #pragma unroll
       for (int q = 0; q < NTHR / 64; ++q) any |= s_nz[q];
@@ -805,9 +810,10 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
           for (int u = 0; u < kSub; ++u) {
             const int j = j0 + u * G + g;
             const SrcPt<R> sp = s_p[j];
-            const R sx = sp.x, sy = sp.y, sz = sp.z;
+            const R sx = sp.x, sy = sp.y;
+            const R sz = kPacked && kPlanar ? (R)0 : sp.z;
             const R sa = sp.a;
-            wv[u] = s_w[j];
+            wv[u] = kPacked && kPlanar ? sp.z : s_w[j];
             px[u] = sx; py[u] = sy; pz[u] = sz;
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
