@@ -66,8 +66,10 @@ struct Seg {
 
 __global__ __launch_bounds__(kReduceThreads) void batch_reduce_kernel(const ReduceSpec* specs,
                                                             const int32_t* act, double* out) {
+  // workgroup (i, q): reduction q of active instance i
   const int32_t k = act[blockIdx.x];
-  reduce_block(specs[k], out + 8 * (int64_t)k);
+  if ((int)blockIdx.y >= specs[k].nops) return;
+  reduce_op(specs[k], blockIdx.y, out + 8 * (int64_t)k);
 }
 
 // LAMP scalar schedule of one iteration (MirrorProxKernel.jl:1073, 1108-1115;
@@ -385,10 +387,11 @@ struct Engine {
     return CULAMP_OK;
   }
 
-  int reduce(const ReduceSpec* d_specs, double* d_out) {
-    if (active.empty()) return CULAMP_OK;
-    hipLaunchKernelGGL(batch_reduce_kernel, dim3((unsigned)active.size()), dim3(kReduceThreads), 0,
-                       ctx->stream, d_specs, d_active.p, d_out);
+  // nops: the largest reduction count of the specs (one workgroup each)
+  int reduce(const ReduceSpec* d_specs, double* d_out, int nops) {
+    if (active.empty() || nops <= 0) return CULAMP_OK;
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3((unsigned)active.size(), (unsigned)nops),
+                       dim3(kReduceThreads), 0, ctx->stream, d_specs, d_active.p, d_out);
     HIP_TRY(hipGetLastError());
     return CULAMP_OK;
   }
@@ -457,7 +460,7 @@ int batch_winf(Engine& E, double p, double* tmp_n, std::vector<double>& winf) {
   TRY(d_out.reserve(8 * (size_t)b->count));
   TRY(E.prepare(CULAMP_FP64, KIND_MAX, true));
   TRY(E.launch(d_dm.p, CULAMP_FP64, KIND_MAX, true, p));
-  TRY(E.reduce(d_rs.p, d_out.p));
+  TRY(E.reduce(d_rs.p, d_out.p, 1));
   std::vector<double> h(8 * (size_t)b->count);
   HIP_TRY(hipMemcpyAsync(h.data(), d_out.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -552,7 +555,7 @@ int batch_sinkhorn(culamp_ctx* ctx, culamp_batch* b, const culamp_args* a, int64
   };
   auto resid = [&]() -> int {
     TRY(E.launch(D3.p, prec, KIND_SUM, true, p_aux));
-    TRY(E.reduce(DR.p, d_out));
+    TRY(E.reduce(DR.p, d_out, 4));
     HIP_TRY(hipMemcpyAsync(h.data(), d_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (int32_t k : E.active) npass[k] += 1;
@@ -787,10 +790,10 @@ int batch_lamp(culamp_ctx* ctx, culamp_batch* b, const culamp_args* a, int64_t f
   auto check = [&](int64_t it, std::vector<int32_t>& still) -> int {
     TRY(E.launch(DE1.p, prec, KIND_LSE, true, pp));
     TRY(E.launch(DE2.p, prec, KIND_SUM, false, pp));
-    TRY(E.reduce(DR1.p, d_out));
+    TRY(E.reduce(DR1.p, d_out, 4));
     if (dual_lse) TRY(E.launch(DE3.p, prec, KIND_LSE, true, pp));
     else TRY(E.launch(DE3.p, CULAMP_FP64, KIND_MIN, true, pp));
-    TRY(E.reduce(DR2.p, d_out + 8 * (int64_t)K));
+    TRY(E.reduce(DR2.p, d_out + 8 * (int64_t)K, 2));
     HIP_TRY(hipMemcpyAsync(h.data(), d_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     still.clear();
@@ -890,7 +893,7 @@ int batch_lamp(culamp_ctx* ctx, culamp_batch* b, const culamp_args* a, int64_t f
     }
     DevArr<ReduceSpec> DRO;
     TRY(DRO.upload(ro, ctx->stream));
-    TRY(E.reduce(DRO.p, d_out));
+    TRY(E.reduce(DRO.p, d_out, 1));
     HIP_TRY(hipMemcpyAsync(h.data(), d_out, sizeof(double) * 8 * K, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (int32_t k = 0; k < K; ++k) objf[k] = h[8 * (size_t)k];

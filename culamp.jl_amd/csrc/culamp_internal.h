@@ -235,8 +235,6 @@ hipError_t launch_batch_fp32(const BatchLaunch& b, int kind, const Work& w, hipS
 hipError_t launch_batch_fp64(const BatchLaunch& b, int kind, const Work& w, hipStream_t s);
 hipError_t launch_batch_mfma(const BatchLaunch& b, int kind, const Work& w, hipStream_t s);
 
-// Body of the one-block reductions: 1024 threads, strided partial sums, then
-// a fixed tree, so the result depends only on the input.
 // Threads of a reduction workgroup.
 constexpr int kReduceThreads = 1024;
 
@@ -251,59 +249,54 @@ __device__ __forceinline__ double reduce_fold(const int op, const double acc, co
   return acc + fabs(x);
 }
 
-__device__ __forceinline__ void reduce_block(const ReduceSpec& spec, double* out) {
-  // One operation at a time (a compact loop: the kernel runs once per check,
-  // mostly from a cold instruction cache), each thread folding four strided
-  // elements per round with their loads in flight together; the partials
-  // meet by butterfly within each wave and across the waves through LDS, one
-  // barrier for all operations.
+// Reduction k of `spec` by one workgroup (the reductions of a spec run as
+// separate workgroups, so their memory round trips overlap): each thread folds
+// four strided elements per round with their loads in flight together, the
+// partials meet by butterfly within each wave and across the waves through
+// LDS; a fixed order, so the result depends only on the input.
+__device__ __forceinline__ void reduce_op(const ReduceSpec& spec, const int k, double* out) {
   constexpr int W = kReduceThreads / 64;
   constexpr int U = 4;
-  __shared__ double sh[8][W];
+  __shared__ double sh[W];
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-  for (int k = 0; k < spec.nops; ++k) {
-    const int op = spec.op[k];
-    const double* a = spec.a[k];
-    const double* b = spec.b[k];
-    const int64_t n = spec.len[k];
-    const bool two = op == 1 || op == 2;
-    double v = op == 3 ? -INFINITY : 0.0;
-    for (int64_t i0 = tid; i0 < n; i0 += U * kReduceThreads) {
-      double xa[U], xb[U];
+  const int op = spec.op[k];
+  const double* a = spec.a[k];
+  const double* b = spec.b[k];
+  const int64_t n = spec.len[k];
+  const bool two = op == 1 || op == 2;
+  const bool mx = op == 3;
+  double v = mx ? -INFINITY : 0.0;
+  for (int64_t i0 = tid; i0 < n; i0 += U * kReduceThreads) {
+    double xa[U], xb[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t i = i0 + (int64_t)u * kReduceThreads;
-        xa[u] = 0.0;
-        xb[u] = 0.0;
-        if (i < n) {
-          xa[u] = a[i];
-          if (two) xb[u] = b[i];
-        }
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + (int64_t)u * kReduceThreads;
+      xa[u] = 0.0;
+      xb[u] = 0.0;
+      if (i < n) {
+        xa[u] = a[i];
+        if (two) xb[u] = b[i];
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (i0 + (int64_t)u * kReduceThreads < n) v = reduce_fold(op, v, xa[u], xb[u]);
     }
-    const bool mx = op == 3;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double o = __shfl_xor(v, off);
-      v = mx ? fmax(v, o) : v + o;
-    }
-    if (lane == 0) sh[k][wave] = v;
+    for (int u = 0; u < U; ++u)
+      if (i0 + (int64_t)u * kReduceThreads < n) v = reduce_fold(op, v, xa[u], xb[u]);
   }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off);
+    v = mx ? fmax(v, o) : v + o;
+  }
+  if (lane == 0) sh[wave] = v;
   __syncthreads();
   if (wave != 0) return;
-  for (int k = 0; k < spec.nops; ++k) {
-    const bool mx = spec.op[k] == 3;
-    double v = lane < W ? sh[k][lane] : (mx ? -INFINITY : 0.0);
+  v = lane < W ? sh[lane] : (mx ? -INFINITY : 0.0);
 #pragma unroll
-    for (int off = W / 2; off > 0; off >>= 1) {
-      const double o = __shfl_xor(v, off);
-      v = mx ? fmax(v, o) : v + o;
-    }
-    if (lane == 0) out[k] = v;
+  for (int off = W / 2; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off);
+    v = mx ? fmax(v, o) : v + o;
   }
+  if (lane == 0) out[k] = v;
 }
 
 // out = sa * a + sb * b with both products rounded (no fused multiply-add),

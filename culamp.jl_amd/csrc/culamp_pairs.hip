@@ -10,8 +10,9 @@ namespace {
 
 constexpr int kSub = 8;  // must match the pair kernel's sub-tile
 
+// one workgroup per reduction of the spec
 __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(const ReduceSpec spec, double* out) {
-  reduce_block(spec, out);
+  reduce_op(spec, blockIdx.x, out);
 }
 
 __global__ void axpby_kernel(double* out, const double* a, double sa,
@@ -179,7 +180,9 @@ LaunchCfg choose_mfma_launch(int64_t nt, int64_t ns, int num_cu, int force_split
 }
 
 hipError_t launch_reduce(const ReduceSpec& spec, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kReduceThreads), 0, s, spec, out);
+  if (spec.nops <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)spec.nops), dim3(kReduceThreads), 0, s, spec,
+                     out);
   return hipGetLastError();
 }
 
