@@ -800,8 +800,9 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
     } else {
       // lane g takes sources j0 + u*G + g: the G lanes of a target read G
       // consecutive LDS words per step
-      auto tile_pairs = [&](auto planar_tag) {
+      auto tile_pairs = [&](auto planar_tag, auto cost_tag) {
         constexpr bool kPlanar = decltype(planar_tag)::value;
+        constexpr bool kNeedC = decltype(cost_tag)::value;
         for (int j0 = 0; j0 < cntp; j0 += kSub * G) {
           R v[RPT][kSub];
           R Cv[RPT][kSub];
@@ -838,15 +839,25 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
 #pragma unroll
           for (int r = 0; r < RPT; ++r) {
             if constexpr (kLazy) lse_lazy<kSub>(st[r], v[r], wv, tq[r], tqm[r], dl[r]);
-            else accumulate<PREC, KIND, kSub, R>(st[r], v[r], wv, Cv[r], px, py, pz);
+            else accumulate<PREC, KIND, kSub, R, kNeedC>(st[r], v[r], wv, Cv[r], px, py, pz);
           }
         }
       };
+      // A marginal pass accumulates the cost sum only when its cost output is
+      // asked for (the sum feeds nothing else; the marginals keep their bits).
+      auto with_cost = [&](auto planar_tag) {
+        if constexpr (KIND == KIND_SUM) {
+          if (d.out1 != nullptr) tile_pairs(planar_tag, std::true_type{});
+          else tile_pairs(planar_tag, std::false_type{});
+        } else {
+          tile_pairs(planar_tag, std::true_type{});
+        }
+      };
       if constexpr (kPlanarOk) {
-        if (planar) tile_pairs(std::true_type{});
-        else tile_pairs(std::false_type{});
+        if (planar) with_cost(std::true_type{});
+        else with_cost(std::false_type{});
       } else {
-        tile_pairs(std::false_type{});
+        with_cost(std::false_type{});
       }
     }
   }
