@@ -271,6 +271,40 @@ def test_planar_tiles_match_general_loop(gpu_ctx, n, m):
                 assert np.array_equal(w, g), (prec, label, k)
 
 
+@pytest.mark.parametrize("n,m", [(4096, 4096), (3000, 5000), (70, 33)])
+def test_marginals_same_without_cost(gpu_ctx, n, m):
+    """A marginal pass asked for no cost output skips the cost sum; its
+    marginals (LAMP column pass, Sinkhorn row residual) keep their bits, for
+    planar and 3-D points, FP64 and mixed, every cost."""
+    from culamp_amd import FP64, MIXED
+    from culamp_amd import kernels as K
+    say("marginals without cost n", n, "m", m)
+    rng = np.random.default_rng(7 * n + m)
+    x, y = rng.uniform(size=(n, 3)), rng.uniform(size=(m, 3))
+    th = rng.uniform(-1, 1, m)
+    r = rng.uniform(0.1, 1, n)
+    r /= r.sum()
+    dth, dr = dev(gpu_ctx, th), dev(gpu_ctx, r)
+    dlz, dphi = dev(gpu_ctx, rng.uniform(-1, 2, n)), dev(gpu_ctx, rng.normal(0, 1, n))
+    res, cost, rres, rcost = out(gpu_ctx, m), out(gpu_ctx, m), out(gpu_ctx, n), out(gpu_ctx, n)
+    for planar in (True, False):
+        xa, ya = x.copy(), y.copy()
+        if planar:
+            xa[:, 2] = 0.0
+            ya[:, 2] = 0.0
+        dx, dy = dev(gpu_ctx, xa), dev(gpu_ctx, ya)
+        for p in (2.0, 1.0, math.inf):
+            for prec in (FP64, MIXED):
+                K.residual_spp_c(gpu_ctx, res, cost, dx, dy, dr, dth, dlz, 0.02, 1.0, 3.0, p, prec)
+                want = res.get()
+                K.residual_spp_c(gpu_ctx, res, None, dx, dy, dr, dth, dlz, 0.02, 1.0, 3.0, p, prec)
+                assert np.array_equal(res.get(), want), (planar, p, prec)
+                K.residual_opt(gpu_ctx, rres, rcost, dx, dy, dr, dphi, dth, 0.02, 3.0, p, prec)
+                want = rres.get()
+                K.residual_opt(gpu_ctx, rres, None, dx, dy, dr, dphi, dth, 0.02, 3.0, p, prec)
+                assert np.array_equal(rres.get(), want), (planar, p, prec)
+
+
 def test_captured_graph_replays_match_eager(gpu_ctx, oracle):
     """Kernel-level calls captured into a graph: every replay equals the eager
     result, with new inputs uploaded and eager work run between the replays."""
