@@ -545,11 +545,20 @@ def main() -> None:
     # the compute side: one v_exp_f32 per pair, against the device's measured
     # exponential throughput (independent-chain micro-kernel)
     rates = ctx.probe_rates()
+    # The pair loop's own VALU bound: in the gfx950 assembly of the planar
+    # mixed row LSE a sub-tile of 8 pairs per lane is 63 VALU instructions,
+    # v_exp_f32 (half rate) taking two issue slots: (63 + 8) / 8 slots per
+    # pair at the measured FMA issue rate.
+    slots_per_pair = (63 + 8) / 8
+    loop_bound = rates["fma_per_s"] / slots_per_pair
     roofline["compute"] = {
         "bound": "v_exp_f32 + VALU", "unit": "exp/s",
         "achieved": roofline["pairs_per_s"], "peak_measured": rates["exp_per_s"],
         "frac": roofline["pairs_per_s"] / rates["exp_per_s"],
-        "fma_per_s_measured": rates["fma_per_s"]}
+        "fma_per_s_measured": rates["fma_per_s"],
+        "pair_loop_slots_per_pair": slots_per_pair,
+        "pair_loop_bound_pairs_per_s": loop_bound,
+        "frac_of_pair_loop_bound": roofline["pairs_per_s"] / loop_bound}
     log(f"probe: {rates['exp_per_s'] / 1e12:.2f} T exp/s, {rates['fma_per_s'] / 1e12:.2f} T fma/s; "
         f"row-LSE pass {roofline['pairs_per_s'] / 1e12:.2f} T pairs/s")
 
