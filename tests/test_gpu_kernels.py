@@ -171,16 +171,22 @@ def test_deterministic_repeat(gpu_ctx, oracle):
             assert np.array_equal(a, o.get())
 
 
+@pytest.mark.parametrize("planar", [False, True])
 @pytest.mark.parametrize("n,m", [(4096, 4096), (4100, 333), (700, 5000), (16, 9), (1, 7)])
-def test_wide_layout_matches_split_layout(gpu_ctx, oracle, n, m):
+def test_wide_layout_matches_split_layout(gpu_ctx, oracle, n, m, planar):
     """Source splits merged inside one workgroup (wide layout) and over
     workgroups with a merge launch (split layout) give identical bits for the
     same split length, for every kind the wide layout runs, both precisions
-    and every cost; the wide FP64 row LSE also matches the oracle."""
+    and every cost, for 3-D points and for points with z = 0 (whose wide
+    tiles carry the weight in the z slot); the wide FP64 row LSE also matches
+    the oracle."""
     from culamp_amd import FP64, MIXED
     from culamp_amd import kernels as K
-    say("wide vs split n", n, "m", m)
+    say("wide vs split n", n, "m", m, "planar", planar)
     x, y, th, r = _inputs(3 * n + m, n, m)
+    if planar:
+        x[:, 2] = 0.0
+        y[:, 2] = 0.0
     dx, dy, dth, dr = dev(gpu_ctx, x), dev(gpu_ctx, y), dev(gpu_ctx, th), dev(gpu_ctx, r)
     lz = dev(gpu_ctx, np.linspace(-1.0, 2.0, n))
     o, res, cost = out(gpu_ctx, n), out(gpu_ctx, m), out(gpu_ctx, m)
