@@ -176,7 +176,8 @@ int culamp_lse_rows(culamp_ctx *ctx, const double *x, int64_t n,
                     int precision, double *out);
 /* residual_spp_c! (MirrorProxKernel.jl:7-94): for each column j
  *   w_ij = r[i] exp(-(C_ij st/(2 winf) + theta[j]) / reg - logZ[i])
- *   res[j] = sum_i w_ij,  cost[j] = sum_i w_ij C_ij   (cost may be NULL). */
+ *   res[j] = sum_i w_ij,  cost[j] = sum_i w_ij C_ij   (cost may be NULL;
+ *   the pass then skips the cost sum and res keeps its bits). */
 int culamp_col_marginal_cost(culamp_ctx *ctx, const double *x, int64_t n,
                              const double *y, int64_t m, const double *r,
                              const double *theta, const double *logZ,
@@ -205,7 +206,8 @@ int culamp_sinkhorn_half_step(culamp_ctx *ctx, const double *x, int64_t n,
                               double p, int precision, double *out);
 /* residual_opt! (SinkhornKernel.jl:9-81):
  *   w_ij = exp(-C_ij/(reg winf) + phi[i] + psi[j])
- *   res[i] = marg[i] - sum_j w_ij,  cost[i] = sum_j w_ij C_ij (may be NULL). */
+ *   res[i] = marg[i] - sum_j w_ij,  cost[i] = sum_j w_ij C_ij (may be NULL;
+ *   the pass then skips the cost sum and res keeps its bits). */
 int culamp_row_residual_cost(culamp_ctx *ctx, const double *x, int64_t n,
                              const double *y, int64_t m, const double *marg,
                              const double *phi, const double *psi, double reg,
