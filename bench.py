@@ -565,7 +565,7 @@ def main() -> None:
     cpu = None
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle  # the checker, timed here as the CPU baseline
+        import oracle  # the FP64 CPU oracle, timed here as the CPU baseline
         threads = min(os.cpu_count() or 1, 16)
         oracle.set_threads(threads)
         tc = time.perf_counter()

@@ -1,7 +1,8 @@
 /*
  * culamp_oracle.c -- FP64 CPU restatement of CuLAMP.jl's hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker; never part of the product path).
+ * TEST INFRASTRUCTURE ONLY (the results are checked against it; never part
+ * of the product path).
  * See culamp_oracle.h for conventions.  Each function cites the reference
  * file:line it restates.  Reductions over sources are sequential in index
  * order; log-sum-exps are two-pass (max, then sum of exp(v - max)), which is

@@ -2,9 +2,8 @@
  * culamp_oracle.h -- FP64 CPU restatement of CuLAMP.jl's on-the-fly cost +
  * log-sum-exp hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests, the smoke
- * check and the CPU-baseline leg of the benchmark load it to compare against
- * the HIP library.  It is never linked into, called from, or used as a
+ * TEST INFRASTRUCTURE ONLY.  Tests, the smoke check and the CPU-baseline leg
+ * of the benchmark load this library to compare the HIP library against it.  It is never linked into, called from, or used as a
  * fallback by the product library (libculamp).
  *
  * Every function states the reference file:line it restates (paths relative to

@@ -1,7 +1,7 @@
 """ctypes loader for the FP64 CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker.  The product library never
+cpu_baseline leg of bench.py, to check results.  The product library never
 loads it.
 
 Arrays follow the oracle's C conventions: points (n, 3) C-contiguous float64

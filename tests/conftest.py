@@ -34,7 +34,7 @@ def golden():
 
 @pytest.fixture(scope="session")
 def oracle():
-    import oracle as o  # oracle/oracle.py (the checker)
+    import oracle as o  # oracle/oracle.py (the FP64 CPU oracle)
     o.lib()
     o.set_threads(min(os.cpu_count() or 1, 16))
     return o
