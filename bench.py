@@ -487,8 +487,8 @@ def main() -> None:
         bench_dense(culamp_amd.Context(0), a, lambda *x: print(*x, file=sys.stderr, flush=True))
         return
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
-              "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
+        print(f"bench.py: the library runs on one GPU (asked for --gpus {a.gpus}); "
+              "sharding over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
