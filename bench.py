@@ -464,6 +464,8 @@ def main() -> None:
     ap.add_argument("--clouds-n", type=int, default=16384)
     ap.add_argument("--no-clouds", action="store_true")
     ap.add_argument("--clouds-lamp-iters", type=int, default=200)
+    ap.add_argument("--clouds-only", action="store_true",
+                    help="only the RGB-cloud Sinkhorn / LAMP comparison (profiling)")
     ap.add_argument("--diff-n", type=int, default=65536)
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
@@ -485,6 +487,10 @@ def main() -> None:
     if a.dense_only:
         import culamp_amd
         bench_dense(culamp_amd.Context(0), a, lambda *x: print(*x, file=sys.stderr, flush=True))
+        return
+    if a.clouds_only:
+        import culamp_amd
+        bench_clouds(culamp_amd.Context(0), a, lambda *x: print(*x, file=sys.stderr, flush=True))
         return
     if a.gpus != 1:
         print(f"bench.py: the library runs on one GPU (asked for --gpus {a.gpus}); "
