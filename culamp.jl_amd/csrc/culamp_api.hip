@@ -1282,8 +1282,10 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
         emit(log_fn, user, fmt("%.6e", elapsed) + "," + std::to_string(it) + "," +
                                fmt("%.14e", resid) + "," + fmt("%.14e", objective) + "," +
                                fmt("%.14e", primal) + "," + fmt("%.14e", dual) + ",lamp_kernel\n");
+      // CULAMP_STOP_INFEAS never stops at the pre-step check (eta_t = inf:
+      // the plan is r (x) 1/m whatever the costs, feasible when c is uniform)
       const bool met = a->stop == CULAMP_STOP_INFEAS
-                           ? resid < a->epsilon / 6.0
+                           ? (std::isfinite(etat_prev) && resid < a->epsilon / 6.0)
                            : (primal - dual < a->epsilon / 6.0 && resid < a->epsilon / 6.0);
       if (met) {
         // undo the half step run past the stop: the state is the check's
@@ -1583,7 +1585,7 @@ int culamp_lamp_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_
                                fmt("%.14e", resid) + "," + fmt("%.14e", objective) + "," +
                                fmt("%.14e", primal) + "," + fmt("%.14e", dual) + ",lamp_kernel\n");
       const bool met = a->stop == CULAMP_STOP_INFEAS
-                           ? resid < a->epsilon / 6.0
+                           ? (std::isfinite(etat) && resid < a->epsilon / 6.0)
                            : (primal - dual < a->epsilon / 6.0 && resid < a->epsilon / 6.0);
       if (met) {
         converged = 1;

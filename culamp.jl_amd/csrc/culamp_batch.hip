@@ -819,8 +819,9 @@ int batch_lamp(culamp_ctx* ctx, culamp_batch* b, const culamp_args* a, int64_t f
         if (e > 0) dual[k] = -2.0 * e * w[0] - 2.0 * W * w[1] + 2.0 * hr[k];
         else dual[k] = w[0] - 2.0 * W * w[1];
       }
+      // CULAMP_STOP_INFEAS skips the pre-step check (eta_t = inf)
       const bool met = a->stop == CULAMP_STOP_INFEAS
-                           ? resid[k] < a->epsilon / 6.0
+                           ? (std::isfinite(et) && resid[k] < a->epsilon / 6.0)
                            : (primal[k] - dual[k] < a->epsilon / 6.0 && resid[k] < a->epsilon / 6.0);
       if (met) {
         conv[k] = 1;

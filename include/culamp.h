@@ -76,8 +76,11 @@ typedef struct culamp_dense culamp_dense;
  * behaviour: `precision` (CULAMP_FP64) and `stop` (CULAMP_STOP_REFERENCE).
  * CULAMP_STOP_INFEAS stops LAMP when the L1 infeasibility is below
  * epsilon/6, tested every `frequency` iterations whether or not `verbose` is
- * set (the reference tests primal - dual and infeasibility, and only when
- * verbose, MirrorProxKernel.jl:1081,1098). */
+ * set, except at the first check (eta_t = inf there, so the plan is the
+ * product r x 1/m whatever the costs, and it is feasible whenever c is
+ * uniform).  The reference tests primal - dual and infeasibility, and only
+ * when verbose (MirrorProxKernel.jl:1081,1098); its gap term is what keeps it
+ * from stopping at that first check. */
 typedef struct {
   double eta_p, eta_mu, tau_p, tau_mu, alpha, B, epsilon, anneal_mult;
   int64_t itermax, inner_iter;

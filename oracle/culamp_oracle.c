@@ -364,7 +364,10 @@ int or_lamp_solve(const double *x, int64_t n, const double *y, int64_t m,
       snprintf(line, sizeof line, "%s,%lld,%s,%s,%s,%s,lamp_kernel\n", b0,
                (long long)it, b1, b2, b3, b4);
       if (args->verbose) emit(logfn, user, line);
-      const int met = args->stop == 1 ? resid < args->epsilon / 6.0
+      /* stop == 1: the infeasibility alone, never at the pre-step check
+       * (eta_t = inf: the plan is r (x) 1/m whatever the costs, exactly
+       * feasible when c is uniform) */
+      const int met = args->stop == 1 ? (isfinite(etat) && resid < args->epsilon / 6.0)
                                        : (primal - dual < args->epsilon / 6.0 && resid < args->epsilon / 6.0);
       if (met) {
         converged = 1;
@@ -970,7 +973,10 @@ int or_lamp_cellsim_solve(const double *data, int64_t n, int64_t d,
                  b1, b2, b3, b4);
         emit(logfn, user, line);
       }
-      const int met = args->stop == 1 ? resid < args->epsilon / 6.0
+      /* stop == 1: the infeasibility alone, never at the pre-step check
+       * (eta_t = inf: the plan is r (x) 1/m whatever the costs, exactly
+       * feasible when c is uniform) */
+      const int met = args->stop == 1 ? (isfinite(etat) && resid < args->epsilon / 6.0)
                                        : (primal - dual < args->epsilon / 6.0 && resid < args->epsilon / 6.0);
       if (met) {
         converged = 1;

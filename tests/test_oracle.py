@@ -278,3 +278,26 @@ def test_cellsim_lamp_stop_and_annealing(oracle):
     etas = [float(l.split(",")[4]) for l in a["log"][1:]]
     assert etas == [0.2, 0.1, 0.05]
     assert a["log"][1].rstrip().endswith("annealed_sinkhorn_cellsim")
+
+
+def test_lamp_infeas_stop_skips_product_plan_check(oracle):
+    """With uniform target weights the plan of the first check (eta_t = inf:
+    r (x) 1/m) is exactly feasible; the infeasibility stop must not end there.
+    At eta_p > 0 it then stops where the reference's gap-and-infeasibility
+    stop (MirrorProxKernel.jl:1081-1101) does, with the same state."""
+    from culamp_amd import workloads
+    x, y, r, c = workloads.gmm_clouds(150, 150)
+    assert np.allclose(c, 1.0 / 150)
+    kw = dict(frequency=10, p=2.0, eta_p=1e-2, epsilon=6e-6, B=1.1, itermax=20000)
+    inf = oracle.lamp_solve(x, y, r, c, verbose=False, stop="infeas", **kw)
+    ref = oracle.lamp_solve(x, y, r, c, verbose=True, stop="reference", **kw)
+    prod = float(r @ ((x[:, None, :] - y[None, :, :]) ** 2).sum(-1).mean(1))
+    assert inf["converged"] == 1 and inf["iterations"] > 0 and inf["residual"] < 1e-6
+    assert inf["objective"] < 0.6 * prod
+    assert ref["converged"] == 1 and ref["iterations"] == inf["iterations"]
+    assert ref["objective"] == inf["objective"]
+    assert np.array_equal(ref["nu"], inf["nu"])
+    # eta_p = 0: the reference's stop does not fire (its gap closes slowly);
+    # the infeasibility stop does, after real iterations
+    z = oracle.lamp_solve(x, y, r, c, verbose=False, stop="infeas", **{**kw, "eta_p": 0.0})
+    assert z["converged"] == 1 and z["iterations"] > 0 and z["objective"] < 0.6 * prod
