@@ -21,7 +21,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libculamp.so")
-SOURCES = ["culamp_api.hip", "culamp_batch.hip", "culamp_dense.hip", "culamp_diff.hip", "culamp_pairs.hip",
+SOURCES = ["culamp_api.hip", "culamp_batch.hip", "culamp_dense.hip", "culamp_diff.hip", "culamp_noalloc.hip", "culamp_pairs.hip",
            "culamp_pass_fp32.hip", "culamp_pass_fp64.hip", "culamp_probe.hip"]
 ARCH = os.environ.get("CULAMP_OFFLOAD_ARCH", "gfx950")
 

@@ -567,6 +567,7 @@ int culamp_ctx_create(int device, culamp_ctx** out) {
     culamp_ctx_destroy(ctx);
     return fail(CULAMP_ERR_HIP, "context creation failed: %s", hipGetErrorString(e));
   }
+  culamp_noalloc_setup(device);  // (the device is current)
   *out = ctx;
   return CULAMP_OK;
 }

@@ -92,6 +92,9 @@ int reduce_start(culamp_ctx* ctx, const culamp::ReduceSpec& rs);
 int reduce_finish(culamp_ctx* ctx, int nops, double* out);
 int copy_out(culamp_ctx* ctx, double* host, const double* dev, int64_t n);
 double sk_aux_p(double p);
+// Looks up the device address of the no-allocation entry point's scratch
+// array on `device` (current), once per device.
+void culamp_noalloc_setup(int device);
 
 PassDesc lse_rows_desc(const double* x, int64_t n, const double* y, int64_t m,
                        const double* theta, double reg, double st, double winf,

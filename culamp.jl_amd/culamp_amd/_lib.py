@@ -119,6 +119,7 @@ SIGNATURES = {
     "culamp_graph_destroy": (_I, [_P]),
     "culamp_max_cost": (_I, [_P, _P, _I64, _P, _I64, _D, _P]),
     "culamp_lse_rows": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _D, _D, _D, _I, _P]),
+    "culamp_lse_rows_noalloc": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _D, _D, _D, _I, _P]),
     "culamp_col_marginal_cost": (_I, [_P, _P, _I64, _P, _I64, _P, _P, _P, _D, _D, _D, _D, _I, _P, _P]),
     "culamp_minplus_rows": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _D, _P]),
     "culamp_lamp_theta_update": (_I, [_P, _P, _P, _P, _P, _P, _I64, _D, _I, _D, _D, _D]),

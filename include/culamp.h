@@ -177,6 +177,20 @@ int culamp_lse_rows(culamp_ctx *ctx, const double *x, int64_t n,
                     const double *y, int64_t m, const double *theta,
                     double reg, double st, double winf, double p,
                     int precision, double *out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): culamp_lse_rows with the same arguments, kernels, launch
+ * shape and result bits, except that it calls no hipMalloc / hipFree and no
+ * allocating routine.  Its split partials live in a static 8 MiB device array
+ * (2^20 doubles) that each GPU the library runs on holds until the process
+ * ends; a launch that would need more (only a forced shape can, see
+ * culamp_ctx_set_launch) is rejected with CULAMP_ERR_INVALID_ARG.  The array
+ * is shared by every call on one GPU and keeps the last call's contents:
+ * calls that overlap on one GPU (several contexts or threads) race. */
+int culamp_lse_rows_noalloc(culamp_ctx *ctx, const double *x, int64_t n,
+                            const double *y, int64_t m, const double *theta,
+                            double reg, double st, double winf, double p,
+                            int precision, double *out);
 /* residual_spp_c! (MirrorProxKernel.jl:7-94): for each column j
  *   w_ij = r[i] exp(-(C_ij st/(2 winf) + theta[j]) / reg - logZ[i])
  *   res[j] = sum_i w_ij,  cost[j] = sum_i w_ij C_ij   (cost may be NULL;
