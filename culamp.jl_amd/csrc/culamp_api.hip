@@ -379,7 +379,8 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
   const char* tag = cellsim ? ",sinkhorn_cellsim\n" : ",sinkhorn_kernel\n";
   // Staged source records (mixed point problems): each half step's epilogue
   // writes the FP32 source records of the other half step (see PassDesc).
-  const bool use_rec = ctx->records && a->precision == CULAMP_MIXED && !cellsim;
+  PrecisionGuard guard(a->precision, ctx->fp64_fallback != 0);
+  bool use_rec = ctx->records && a->precision == CULAMP_MIXED && !cellsim;
   DevBufs RB;
   float4 *rec_x = nullptr, *rec_y = nullptr;
   float *recw_x = nullptr, *recw_y = nullptr;
@@ -402,7 +403,7 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
       d1.rec_out = rec_x; d1.recw_out = recw_x;
       d1.rec_alpha = nx.alpha; d1.rec_k = nx.k; d1.rec_scaled = rec_scaled; d1.rec_wsrc = nx.srcB;
     }
-    return run_pass(ctx, d1, a->precision, KIND_LSE);
+    return run_pass(ctx, d1, guard.prec, KIND_LSE);
   };
   auto half_cols = [&]() {
     PassDesc d2 = sk_half_desc(pb->y, m, pb->x, n, pb->c, st.phi, eta, winf, p_lse, st.psi);
@@ -414,14 +415,14 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
       d2.rec_out = rec_y; d2.recw_out = recw_y;
       d2.rec_alpha = ny.alpha; d2.rec_k = ny.k; d2.rec_scaled = rec_scaled; d2.rec_wsrc = ny.srcB;
     }
-    const int rc = run_pass(ctx, d2, a->precision, KIND_LSE);
+    const int rc = run_pass(ctx, d2, guard.prec, KIND_LSE);
     recy_ok = use_rec;
     return rc;
   };
   auto resid = [&]() {
     PassDesc d3 = sk_resid_desc(pb->x, n, pb->y, m, pb->r, st.phi, st.psi, eta, winf, p_aux, st.res, st.cost);
     feat(d3, pb, true);
-    return run_pass(ctx, d3, a->precision, KIND_SUM);
+    return run_pass(ctx, d3, guard.prec, KIND_SUM);
   };
   if (a->verbose) emit(log_fn, user, "time(s),iter,infeas,ot_objective,dual\n");
   // A check's sums are read back after the next iteration's half steps are
@@ -476,6 +477,12 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
         if (spec) HIP_TRY(restore());
         break;
       }
+      if (guard.observe(rr, a->epsilon / 6.0, it)) {
+        // later passes in FP64; the half steps already enqueued stay mixed
+        // and still write records, which the next FP64 pass does not read
+        use_rec = false;
+        recy_ok = false;
+      }
       ahead = spec;
     }
     num_iter += 1;
@@ -500,6 +507,7 @@ int sinkhorn_core(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
     out->converged = converged;
     out->nonfinite = !(std::isfinite(v[0]) && std::isfinite(v[1]));
     out->elapsed_s = now_s() - t0;
+    out->fp64_from = guard.fp64_from;
   }
   return CULAMP_OK;
 }
@@ -614,6 +622,12 @@ int culamp_ctx_set_source_records(culamp_ctx* ctx, int enable) {
   TRY(check_ctx(ctx));
   if (enable != 0 && enable != 1) return fail(CULAMP_ERR_INVALID_ARG, "enable must be 0 or 1");
   ctx->records = enable;
+  return CULAMP_OK;
+}
+
+int culamp_ctx_set_fp64_fallback(culamp_ctx* ctx, int enable) {
+  if (!ctx) return fail(CULAMP_ERR_INVALID_ARG, "null context");
+  ctx->fp64_fallback = enable ? 1 : 0;
   return CULAMP_OK;
 }
 
@@ -1115,7 +1129,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
   if (pb->metric >= 0) return fail(CULAMP_ERR_INVALID_ARG, "feature problem: use the cellsim solvers");
   if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
   const int64_t n = pb->n, m = pb->m;
-  const int prec = a->precision;
+  PrecisionGuard guard(a->precision, ctx->fp64_fallback != 0);
   const double pairs0 = ctx->pairs;
   const int64_t passes0 = ctx->passes;
   DevBufs B;
@@ -1148,7 +1162,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
   // column pass over x, the column pass for the next row LSE over y), so
   // those passes skip converting their sources.  recy_ok: rec_y holds the
   // records of lse(nu or nub, etat, st) for the current etat and st.
-  const bool use_rec = ctx->records && prec == CULAMP_MIXED;
+  bool use_rec = ctx->records && a->precision == CULAMP_MIXED;
   float4 *rec_x = nullptr, *rec_y = nullptr;
   float *recw_x = nullptr, *recw_y = nullptr;
   if (use_rec) {
@@ -1178,7 +1192,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
   auto lse = [&](const double* pot, double reg, double s, double* dst, bool offset) {
     PassDesc d = lse_rows_desc(pb->x, n, pb->y, m, pot, reg, s, winf, p, dst);
     d.prev_offset = offset ? 1 : 0;
-    return run_pass(ctx, d, prec, KIND_LSE);
+    return run_pass(ctx, d, guard.prec, KIND_LSE);
   };
   // the row LSE of the loop: lse(pot, etat, st) -> logZ, reading rec_y when
   // it is current and writing rec_x
@@ -1187,7 +1201,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
     d.prev_offset = 1;
     if (use_rec && recy_ok) { d.rec = rec_y; d.recw = recw_y; }
     emit_x(d);
-    return run_pass(ctx, d, prec, KIND_LSE);
+    return run_pass(ctx, d, guard.prec, KIND_LSE);
   };
   auto use_x = [&](PassDesc& d) {
     if (use_rec) { d.rec = rec_x; d.recw = recw_x; }
@@ -1207,7 +1221,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
     if (!with_cost) d.out1 = nullptr;  // the column costs are only read at checks
     use_x(d);
     emit_y(d, etat_new, st_new);  // nub, for the next row LSE
-    return run_pass(ctx, d, prec, KIND_SUM);
+    return run_pass(ctx, d, guard.prec, KIND_SUM);
   };
   for (int64_t it = 1; it <= a->itermax; ++it) {
     const double elapsed = now_s() - t0;
@@ -1267,7 +1281,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
       d.out1 = nullptr;
       use_x(d);
       emit_y(d, etat, st);  // nu, for the next iteration's row LSE
-      TRY(run_pass(ctx, d, prec, KIND_SUM));
+      TRY(run_pass(ctx, d, guard.prec, KIND_SUM));
     }
     if (pending) {
       double v[6];
@@ -1297,6 +1311,10 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
         recy_ok = false;  // rec_y describes the undone state
         break;
       }
+      if (std::isfinite(etat_prev) && guard.observe(resid, a->epsilon / 6.0, it)) {
+        use_rec = false;  // later passes in FP64, which read no records
+        recy_ok = false;
+      }
     }
     steps += 1;
   }
@@ -1305,7 +1323,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
     PassDesc d = lse_rows_desc(pb->x, n, pb->y, m, nu, etat, st, winf, p, logZ);
     d.prev_offset = 1;
     if (use_rec && recy_ok) { d.rec = rec_y; d.recw = recw_y; }
-    TRY(run_pass(ctx, d, prec, KIND_LSE));
+    TRY(run_pass(ctx, d, guard.prec, KIND_LSE));
   }
   double* img_dev = nullptr;
   if (img1 || img2) TRY(B.alloc(&img_dev, 3 * (n > m ? n : m)));
@@ -1344,6 +1362,7 @@ int culamp_lamp_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_args* a,
     out->passes = ctx->passes - passes0;
     out->nonfinite = (a->verbose || a->stop == CULAMP_STOP_INFEAS) &&
                      !(std::isfinite(resid) && std::isfinite(objective));
+    out->fp64_from = guard.fp64_from;
   }
   return CULAMP_OK;
 }
@@ -1426,6 +1445,7 @@ int culamp_annealed_sinkhorn_solve(culamp_ctx* ctx, culamp_problem* pb,
   const int verbose = a.verbose;
   a.verbose = 0;
   int64_t total = 0;
+  int64_t fp64_from = 0;  // first stage's FP64 takeover, in overall iterations
   const double tmax = a.tmax;
   culamp_result r{};
   if (verbose) emit(log_fn, user, "time(s),iter,infeas,ot_objective,eta,solver\n");
@@ -1434,6 +1454,7 @@ int culamp_annealed_sinkhorn_solve(culamp_ctx* ctx, culamp_problem* pb,
     HIP_TRY(hipMemcpyAsync(st.phi, phiws, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(st.psi, psiws, sizeof(double) * m, hipMemcpyDeviceToDevice, ctx->stream));
     TRY(sinkhorn_core(ctx, pb, &a, frequency, p, nullptr, nullptr, st, &r));
+    if (fp64_from == 0 && r.fp64_from > 0) fp64_from = total + r.fp64_from;
     total += r.iterations;
     objective = r.objective;
     residual = r.residual;
@@ -1479,6 +1500,7 @@ int culamp_annealed_sinkhorn_solve(culamp_ctx* ctx, culamp_problem* pb,
     out->eta_t = eta;
     out->elapsed_s = now_s() - t0;
     out->converged = residual <= eta_final / 6.0;
+    out->fp64_from = fp64_from;
     out->pairs = ctx->pairs - pairs0;
     out->passes = ctx->passes - passes0;
     out->nonfinite = !(std::isfinite(residual) && std::isfinite(objective));
@@ -1502,7 +1524,7 @@ int culamp_lamp_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_
   if (!a) return fail(CULAMP_ERR_INVALID_ARG, "null args");
   if (frequency <= 0) return fail(CULAMP_ERR_INVALID_ARG, "frequency must be positive");
   const int64_t n = pb->n, m = pb->m;
-  const int prec = a->precision;
+  PrecisionGuard guard(a->precision, ctx->fp64_fallback != 0);
   const double pairs0 = ctx->pairs;
   const int64_t passes0 = ctx->passes;
   DevBufs B;
@@ -1531,7 +1553,7 @@ int culamp_lamp_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_
     PassDesc d = lse_rows_desc(nullptr, n, nullptr, m, pot, reg, s, winf, 2.0, sumvals);
     feat(d, pb, true);
     d.prev_offset = 1;
-    return run_pass(ctx, d, prec, KIND_LSE);
+    return run_pass(ctx, d, guard.prec, KIND_LSE);
   };
   auto colmarg = [&](const double* pot, double reg, double s, PassDesc* proto) {
     PassDesc d = colmarg_desc(nullptr, n, nullptr, m, pb->r, pot, sumvals, reg, s, winf, 2.0, res, cost);
@@ -1543,7 +1565,7 @@ int culamp_lamp_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_
       d.mix_t = proto->mix_t;
     }
     feat(d, pb, false);
-    return run_pass(ctx, d, prec, KIND_SUM);
+    return run_pass(ctx, d, guard.prec, KIND_SUM);
   };
   for (int64_t it = 1; it <= a->itermax; ++it) {
     const double elapsed = now_s() - t0;
@@ -1592,6 +1614,7 @@ int culamp_lamp_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_
         converged = 1;
         break;
       }
+      if (std::isfinite(etat)) (void)guard.observe(resid, a->epsilon / 6.0, it);
     }
     // 168-180
     const double etat_new = 1.0 / (1.0 + (1.0 / etat) * (1.0 - eta));
@@ -1644,6 +1667,7 @@ int culamp_lamp_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb, const culamp_
     out->pairs = ctx->pairs - pairs0;
     out->passes = ctx->passes - passes0;
     out->nonfinite = !std::isfinite(objf);
+    out->fp64_from = guard.fp64_from;
   }
   return CULAMP_OK;
 }
@@ -1715,11 +1739,13 @@ int culamp_annealed_sinkhorn_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb,
   const int verbose = a.verbose;
   a.verbose = 0;
   int64_t total = 0;
+  int64_t fp64_from = 0;  // first stage's FP64 takeover, in overall iterations
   culamp_result r{};
   if (verbose) emit(log_fn, user, "time(s),iter,infeas,ot_objective,eta,solver\n");
   for (;;) {
     a.eta_p = eta;
     TRY(sinkhorn_core(ctx, pb, &a, frequency, 2.0, nullptr, nullptr, st, &r));
+    if (fp64_from == 0 && r.fp64_from > 0) fp64_from = total + r.fp64_from;
     total += r.iterations;
     if (verbose)
       emit(log_fn, user, fmt("%.6e", now_s() - t0) + "," + std::to_string(total) + "," +
@@ -1740,6 +1766,7 @@ int culamp_annealed_sinkhorn_cellsim_solve(culamp_ctx* ctx, culamp_problem* pb,
   if (out) {
     *out = r;
     out->iterations = total;
+    out->fp64_from = fp64_from;
     out->eta_t = eta;
     out->elapsed_s = now_s() - t0;
     out->pairs = ctx->pairs - pairs0;

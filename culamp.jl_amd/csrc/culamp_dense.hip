@@ -515,6 +515,7 @@ int culamp_dense_lamp_solve(culamp_ctx* ctx, const culamp_dense* dm, const doubl
     out->pairs = ctx->pairs - pairs0;
     out->passes = ctx->passes - passes0;
     out->nonfinite = !(std::isfinite(feas) && std::isfinite(objective));
+    out->fp64_from = 0;
   }
   return CULAMP_OK;
 }

@@ -28,6 +28,7 @@ struct culamp_ctx {
   int force_rpt = 0, force_splits = 0;
   int layout = 0;              // culamp::Layout of point passes
   int records = 1;             // solver loops pass staged source records between passes
+  int fp64_fallback = 1;       // reduced-precision solves hand over to FP64 when they stall
   double pairs = 0.0;          // pair evaluations issued (throughput accounting)
   int64_t passes = 0;
   std::set<culamp_problem*> problems;  // destroyed with the context
@@ -114,6 +115,34 @@ PassDesc max_desc(const double* x, int64_t n, const double* y, int64_t m,
                   double p, double* out);
 PassDesc minplus_desc(const double* x, int64_t n, const double* y, int64_t m,
                       const double* theta, double winf, double p, double* out);
+
+// The FP64 takeover of a reduced-precision solve (culamp_ctx_set_fp64_fallback):
+// fed the infeasibility of every check with a finite eta_t, it switches `prec`
+// to FP64 once the infeasibility has stayed at or above the stop tolerance for
+// kStaleChecks consecutive checks without reaching kGain times its best value.
+struct PrecisionGuard {
+  static constexpr int kStaleChecks = 10;
+  static constexpr double kGain = 0.97;
+  int prec;
+  bool enabled;
+  double best = INFINITY;
+  int stale = 0;
+  int64_t fp64_from = 0;  // iteration of the switch (0: none)
+  PrecisionGuard(int precision, bool on) : prec(precision), enabled(on && precision != CULAMP_FP64) {}
+  // Returns true when the caller's later passes must run in FP64.
+  bool observe(double resid, double tol, int64_t it) {
+    if (!enabled || prec == CULAMP_FP64 || !(resid >= tol)) return false;
+    if (resid < kGain * best) {
+      best = resid;
+      stale = 0;
+      return false;
+    }
+    if (++stale < kStaleChecks) return false;
+    prec = CULAMP_FP64;
+    fp64_from = it;
+    return true;
+  }
+};
 
 // RAII bundle of device buffers for one solve.
 struct DevBufs {

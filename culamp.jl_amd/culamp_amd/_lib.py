@@ -53,7 +53,7 @@ class culamp_result(C.Structure):
         ("residual", C.c_double), ("primal", C.c_double), ("dual", C.c_double),
         ("winf", C.c_double), ("eta_t", C.c_double), ("elapsed_s", C.c_double),
         ("pairs", C.c_double), ("passes", C.c_int64),
-        ("converged", C.c_int32), ("nonfinite", C.c_int32),
+        ("converged", C.c_int32), ("nonfinite", C.c_int32), ("fp64_from", C.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -99,6 +99,7 @@ SIGNATURES = {
     "culamp_ctx_set_launch": (_I, [_P, _I, _I]),
     "culamp_ctx_set_layout": (_I, [_P, _I]),
     "culamp_ctx_set_source_records": (_I, [_P, _I]),
+    "culamp_ctx_set_fp64_fallback": (_I, [_P, _I]),
     "culamp_probe_rates": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "culamp_malloc": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
     "culamp_free": (_I, [_P, _P]),
@@ -263,6 +264,13 @@ class Context:
         records from each pass's epilogue to the next pass (identical results
         either way)."""
         check(self.lib.culamp_ctx_set_source_records(self.handle, 1 if enable else 0))
+
+    def set_fp64_fallback(self, enable: bool = True) -> None:
+        """Whether reduced-precision solves hand their remaining passes to
+        FP64 when the infeasibility stalls above the stop tolerance (10 checks
+        without reaching 0.97 of its best); the result's ``fp64_from`` gives
+        the iteration of the switch (0: none)."""
+        check(self.lib.culamp_ctx_set_fp64_fallback(self.handle, 1 if enable else 0))
 
     def probe_rates(self) -> dict:
         """Measured FP32 exponential and fused multiply-add throughput of the

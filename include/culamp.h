@@ -108,6 +108,10 @@ typedef struct {
   int64_t passes;      /* O(n*m) passes issued */
   int32_t converged;   /* stop test met */
   int32_t nonfinite;   /* a NaN/Inf was seen in the residual or objective */
+  int64_t fp64_from;   /* reduced-precision solves: the iteration from which
+                        * the passes ran in FP64 because the infeasibility
+                        * stopped improving above the stop tolerance (see
+                        * culamp_ctx_set_fp64_fallback); 0 if they never did */
 } culamp_result;
 
 /* Log sink: called synchronously on the calling thread with one CSV line in
@@ -139,6 +143,17 @@ int culamp_ctx_set_layout(culamp_ctx *ctx, int layout);
  * weight) of the pass after it, which then skips converting its sources; 0
  * makes every pass convert its own.  Both give identical results. */
 int culamp_ctx_set_source_records(culamp_ctx *ctx, int enable);
+/* FP64 takeover of reduced-precision solves (no reference counterpart): with
+ * 1 (the default) the single-instance LAMP, cell-similarity LAMP and Sinkhorn
+ * loops (and the annealed stages) in CULAMP_MIXED, _BF16 or _BF16X3 switch
+ * every later pass to FP64 once the L1 infeasibility has stayed above the
+ * stop tolerance epsilon/6 for 10 consecutive checks without reaching 0.97
+ * times its best value so far.  FP32 exponents carry an absolute error of
+ * about 2^-24 times their magnitude, which at small eta_p (C / eta_t of the
+ * transported pairs in the thousands) puts a floor above 1e-6 under the
+ * infeasibility.  culamp_result.fp64_from reports the switch.  0 keeps the
+ * requested precision throughout (to measure that floor). */
+int culamp_ctx_set_fp64_fallback(culamp_ctx *ctx, int enable);
 /* Compute-rate probe (no reference counterpart; measurement): the device's
  * throughput of the FP32 exponential (v_exp_f32, one per pair in every pass)
  * and of the FP32 fused multiply-add, in lane operations per second, from

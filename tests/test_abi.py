@@ -53,7 +53,7 @@ def test_abi_version_and_defaults():
 def test_struct_layout_matches_header():
     from culamp_amd._lib import culamp_args, culamp_result
     assert C.sizeof(culamp_args) == 8 * 11 + 4 * 4
-    assert C.sizeof(culamp_result) == 8 * 10 + 4 * 2
+    assert C.sizeof(culamp_result) == 8 * 11 + 4 * 2
 
 
 def test_no_device_is_an_error_not_a_fallback():

@@ -138,6 +138,67 @@ def test_grid_histograms_lamp_pnorms(gpu_ctx):
     pb.close()
 
 
+@pytest.mark.parametrize("p", [1.0, 2.0, math.inf])
+def test_grid_histograms_lamp_eta_1e4(gpu_ctx, oracle, p):
+    """64 x 64 grid histograms, eta_p = 1e-4 (the small-regularisation half of
+    the 4096-point grid configuration).  FP64 LAMP stops at infeasibility 1e-6
+    and its returned state is checked with the oracle at full size: the row
+    LSE on 64 random rows, and the column marginal over all 4096 columns,
+    whose L1 infeasibility and cost must be the ones the solver reported.
+    Mixed precision: FP32 exponents reach C / eta_t ~ 1e3 here, so the mixed
+    infeasibility has a floor of 1e-6 to 4e-6; the solve with its FP64
+    takeover reaches 1e-6, and after the FP64 solve's iteration count the
+    pure-mixed cost is within 1e-5 of the FP64 cost."""
+    import culamp_amd
+    from culamp_amd import workloads
+    x, y, r, c = workloads.grid_histograms(64)
+    pb = culamp_amd.Problem(x, y, r, c, gpu_ctx)
+    base = culamp_amd.EOTArgs(eta_p=1e-4, epsilon=6e-6, B=1.1, itermax=20000, verbose=False,
+                              stop="infeas")
+    nu, phi, psi, _, _, f = pb.lamp(base.copy(precision="fp64"), 10, p, log=None,
+                                    assignments=False)
+    say("p", p, "fp64", f["iterations"], f["residual"], f["objective"], f["elapsed_s"])
+    assert f["converged"] == 1 and 0 < f["iterations"] and f["residual"] < 1e-6
+    assert f["fp64_from"] == 0
+    # the returned state, from the oracle
+    winf, reg = f["winf"], f["eta_t"]
+    # the schedule 1 / eta_t = (1 - (1 - eta)^K) / eta, eta = eta_p / (2 W_inf)
+    # (MirrorProxKernel.jl:1035, 1108): far above eta after K ~ 1e3 steps
+    eta = 1e-4 / (2 * winf)
+    assert abs(reg * (1 - (1 - eta) ** f["iterations"]) / eta - 1) < 1e-9
+    logz = np.log(r) - phi
+    rows = np.random.default_rng(41).choice(len(x), 64, replace=False)
+    ref = oracle.lse_rows(x[rows], y, nu, reg, 1.0, winf, p)
+    assert np.max(np.abs(logz[rows] - ref) / np.maximum(np.abs(ref), 1.0)) < 1e-11
+    res, cost = oracle.col_marginal_cost(x, y, r, nu, logz, reg, 1.0, winf, p)
+    infeas = float(np.abs(res - c).sum())
+    say("   oracle infeasibility", infeas, "cost", cost.sum())
+    # (|res_j - c_j| cancels: 4096 terms of ~1e-19 rounding each)
+    assert abs(infeas - f["residual"]) < 1e-14
+    assert abs(cost.sum() - f["objective"]) < 1e-11 * f["objective"]
+    # mixed with the FP64 takeover
+    g = pb.lamp(base.copy(precision="mixed"), 10, p, log=None, assignments=False)[-1]
+    say("   mixed+takeover", g["iterations"], g["residual"], g["objective"], "fp64 from",
+        g["fp64_from"], g["elapsed_s"])
+    assert g["converged"] == 1 and g["residual"] < 1e-6
+    if g["fp64_from"]:
+        assert 0 < g["fp64_from"] <= g["iterations"] + 1
+    # pure mixed after the FP64 solve's iterations (the check at it = K + 1)
+    k = f["iterations"]
+    fixed = base.copy(stop="reference", verbose=True, epsilon=0.0, itermax=k + 1)
+    gpu_ctx.set_fp64_fallback(False)
+    try:
+        h = pb.lamp(fixed.copy(precision="mixed"), 10, p, log=None, assignments=False)[-1]
+    finally:
+        gpu_ctx.set_fp64_fallback(True)
+    say("   mixed after", k, "iterations:", h["objective"], h["residual"], "cost rel err",
+        abs(h["objective"] - f["objective"]) / f["objective"])
+    assert h["fp64_from"] == 0
+    assert abs(h["objective"] - f["objective"]) / f["objective"] < 1e-5
+    assert h["residual"] < 1e-5
+    pb.close()
+
+
 def test_uniform_grid_sinkhorn_vs_dense_reference(gpu_ctx, oracle):
     """32 x 32 grid (n = m = 1024), the reference's CPU path: dense log-domain
     Sinkhorn on W / max W (src/solvers/Sinkhorn.jl:8-84) and the on-the-fly
