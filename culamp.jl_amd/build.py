@@ -33,6 +33,11 @@ CFLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
+# Per-file additions: the MFMA feature kernels (instantiated in the FP32 pass
+# unit) write their accumulators to VGPRs, which their exp-sum reads
+# directly, instead of AGPRs that would need a v_accvgpr_read per value.
+EXTRA = {"culamp_pass_fp32.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
 _NOISE = ("packed-fp32-ops' is not a recognized feature", "packed-fp32-ops\" is not a recognized")
 
 
@@ -47,7 +52,7 @@ def _run(cmd: list[str]) -> None:
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, src.replace(".hip", ".o"))
-    _run([HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    _run([HIPCC, *CFLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj])
     return obj
 
 

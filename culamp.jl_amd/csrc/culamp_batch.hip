@@ -173,6 +173,10 @@ struct Plan {
   size_t partial = 0;  // doubles of split partials
 };
 
+// XCDs of the device: gfx950 has 32 CUs per XCD (256 CUs in 8 XCDs when
+// the whole device is one partition).
+int xcd_count(int num_cu) { return num_cu >= 64 ? num_cu / 32 : 1; }
+
 int targets_per_block(int family, int rpt) {
   return family == FAMILY_POINTS ? (kThreads / kLanesPerTarget) * rpt
                                  : family == FAMILY_FEATURES ? kFeatTargets : 128;
@@ -257,19 +261,50 @@ struct Engine {
     }
     std::stable_sort(order.begin(), order.end(),
                      [&](int32_t a, int32_t c) { return item_work[a] > item_work[c]; });
+    // XCD-aware table: workgroup i of a launch runs on XCD i mod X and each
+    // XCD has its own L2, so all items of an instance go to one XCD, where
+    // they run side by side and read the instance's sources from that L2
+    // (spread over every XCD, each instance's sources were fetched from
+    // memory once per XCD, and the many instances in flight per XCD evicted
+    // each other).  Instances are assigned longest-processing-time first by
+    // total work; each XCD's sequence keeps the largest-item-first order and
+    // shorter sequences are padded with items that do nothing.
+    const int X = xcd_count(ctx->num_cu);
+    std::vector<int32_t> by_work = order;
+    auto work_of = [&](int32_t k) {
+      const BatchInst& bi = P.insts[k];
+      return (double)(bi.ntp / tpb) * (double)bi.splits * (double)bi.split_len;
+    };
+    std::stable_sort(by_work.begin(), by_work.end(),
+                     [&](int32_t a, int32_t c) { return work_of(a) > work_of(c); });
+    std::vector<int> home(b->count, 0);
+    std::vector<double> load(X, 0.0);
+    for (int32_t k : by_work) {
+      int x = 0;
+      for (int q = 1; q < X; ++q)
+        if (load[q] < load[x]) x = q;
+      home[k] = x;
+      load[x] += work_of(k);
+    }
+    std::vector<std::vector<BatchItem>> seq(X);
     P.items.clear();
     P.mitems.clear();
     for (int32_t k : order) {
       const BatchInst& bi = P.insts[k];
       const int64_t tblocks = bi.ntp / tpb;
       for (int32_t sp = 0; sp < bi.splits; ++sp)
-        for (int64_t tb = 0; tb < tblocks; ++tb) P.items.push_back(BatchItem{k, sp, tb});
+        for (int64_t tb = 0; tb < tblocks; ++tb) seq[home[k]].push_back(BatchItem{k, sp, tb});
       if (bi.splits > 1) {
         const int64_t per = kThreads / kMergeLanes;
         const int64_t mb = (nt_of(k, P.rows) + per - 1) / per;
         for (int64_t t = 0; t < mb; ++t) P.mitems.push_back(BatchItem{k, 0, t});
       }
     }
+    size_t longest = 0;
+    for (const auto& q : seq) longest = std::max(longest, q.size());
+    P.items.assign(longest * (size_t)X, BatchItem{-1, 0, 0});
+    for (int x = 0; x < X; ++x)
+      for (size_t j = 0; j < seq[x].size(); ++j) P.items[j * (size_t)X + (size_t)x] = seq[x][j];
     P.partial = (size_t)pbase;
   }
 
@@ -1119,6 +1154,51 @@ int culamp_batch_autotune(culamp_ctx* ctx, culamp_batch* b, int precision, doubl
   b->tune = best_t;
   if (chosen) *chosen = best_t;
   if (ms_per_pass) *ms_per_pass = best;
+  return CULAMP_OK;
+}
+
+int culamp_batch_time_pass(culamp_ctx* ctx, culamp_batch* b, int precision, double p, int reps,
+                           double* ms_per_pass, int32_t* launches_per_pass) {
+  TRY(check_ctx(ctx));
+  if (!b || b->ctx != ctx) return fail(CULAMP_ERR_INVALID_ARG, "null batch or batch of another context");
+  if (precision < CULAMP_FP64 || precision > CULAMP_BF16X3)
+    return fail(CULAMP_ERR_INVALID_ARG, "precision must be CULAMP_FP64, _MIXED, _BF16 or _BF16X3");
+  if (reps <= 0) return fail(CULAMP_ERR_INVALID_ARG, "reps must be positive");
+  const int32_t K = b->count;
+  DevBufs bufs;
+  double *theta, *out;
+  TRY(bufs.alloc(&theta, b->sum_m));
+  TRY(bufs.alloc(&out, b->sum_n));
+  hipLaunchKernelGGL(pooled_fill_kernel, dim3(blocks_of(b->sum_m)), dim3(256), 0, ctx->stream, theta, 0.0, b->sum_m);
+  hipLaunchKernelGGL(pooled_fill_kernel, dim3(blocks_of(b->sum_n)), dim3(256), 0, ctx->stream, out, 0.0, b->sum_n);
+  HIP_TRY(hipGetLastError());
+  const double pp = b->feature ? 2.0 : p;
+  std::vector<PassDesc> hd(K);
+  for (int32_t k = 0; k < K; ++k) {
+    culamp_problem* pb = b->pbs[k];
+    PassDesc d = lse_rows_desc(pb->x, pb->n, pb->y, pb->m, theta + b->off_m[k], 0.05, 1.0, 1.0, pp,
+                               out + b->off_n[k]);
+    feat(d, pb, true);
+    hd[k] = d;
+  }
+  DevArr<PassDesc> D;
+  TRY(upload_descs(D, hd, ctx->stream));
+  Engine E(ctx, b);
+  TRY(E.set_active(all_instances(b)));
+  TRY(E.prepare(precision, KIND_LSE, true));
+  TRY(E.launch(D.p, precision, KIND_LSE, true, pp));  // warm-up
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  for (int q = 0; q < reps; ++q) TRY(E.launch(D.p, precision, KIND_LSE, true, pp));
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(hipEventSynchronize(ctx->ev1));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (ms_per_pass) *ms_per_pass = (double)ms / reps;
+  if (launches_per_pass) {
+    const Engine::Route r = E.route(precision, KIND_LSE);
+    const Plan* P = E.find(r.family, true, r.rpt);
+    *launches_per_pass = (P && !P->mitems.empty()) ? 2 : 1;
+  }
   return CULAMP_OK;
 }
 

@@ -120,7 +120,7 @@ enum DynKind : int { DYN_NONE = 0, DYN_LSE = 1, DYN_COL = 2 };
 // Batched launches: one workgroup per item; instances carry their own launch
 // shape and partial-buffer base.
 struct BatchItem {
-  int32_t inst;
+  int32_t inst;  // < 0: padding, the workgroup does nothing
   int32_t split;
   int64_t tb;
 };

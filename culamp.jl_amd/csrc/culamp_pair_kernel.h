@@ -446,6 +446,7 @@ __global__ __launch_bounds__(kThreads) void pass_merge_batch(const PassDesc* des
                                                              const BatchInst* insts,
                                                              const Work w) {
   const BatchItem it = items[blockIdx.x];
+  if (it.inst < 0) return;  // padding of an XCD's item sequence
   const BatchInst bi = insts[it.inst];
   const PassDesc d = resolve_desc(descs[it.inst]);
   const Work wi{w.partial + bi.pbase};
@@ -985,6 +986,7 @@ __global__ __launch_bounds__(kThreads) void pair_pass_batch(const PassDesc* desc
                                                             const BatchInst* insts,
                                                             const Work w) {
   const BatchItem it = items[blockIdx.x];
+  if (it.inst < 0) return;  // padding of an XCD's item sequence
   const BatchInst bi = insts[it.inst];
   const PassDesc d = resolve_desc(descs[it.inst]);
   const Work wi{w.partial + bi.pbase};
@@ -1237,6 +1239,7 @@ __global__ __launch_bounds__(kThreads) void feature_pass_batch(const PassDesc* d
                                                                const BatchInst* insts,
                                                                const Work w) {
   const BatchItem it = items[blockIdx.x];
+  if (it.inst < 0) return;  // padding of an XCD's item sequence
   const BatchInst bi = insts[it.inst];
   const PassDesc d = resolve_desc(descs[it.inst]);
   const Work wi{w.partial + bi.pbase};
@@ -1284,16 +1287,23 @@ template <int KIND, int NSPLIT, int KS>
 __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, const int64_t tb,
                                           const int split, const int splits,
                                           const int64_t split_len, const int64_t ntp) {
-  constexpr int DP = 16 * KS;          // padded row width
-  constexpr int LDW = DP + 8;          // LDS row stride (bf16), breaks bank alignment
+  constexpr int DP = 16 * KS;          // padded row width (bf16)
+  constexpr int CPR = 2 * KS;          // 16-byte chunks per row
   constexpr int SUB = 32;              // sources per MFMA tile
-  // sources staged per workgroup barrier: several MFMA tiles, so that the
-  // staging round trip and the two barriers are paid once per 2-4 tiles
-  constexpr int TILE = (NSPLIT == 3 || KS > 8) ? 64 : 128;
-  __shared__ __bf16 Ah[TILE * LDW];
-  __shared__ __bf16 Al[NSPLIT == 3 ? TILE * LDW : 1];
-  __shared__ __align__(16) float s_a[TILE];
-  __shared__ __align__(16) float s_w[TILE];
+  // sources staged per workgroup barrier (two MFMA tiles): 2 x 32 KB of
+  // rows per workgroup at d = 128, so that LDS leaves room for three
+  // workgroups per CU
+  constexpr int TILE = 64;
+  // Double-buffered staging: the rows of tile T + 1 go from global memory
+  // straight into the other LDS buffer (global_load_lds, no registers) while
+  // tile T is computed, and the tile's one barrier waits for them.  Rows are
+  // unpadded with their 16-byte chunks rotated by an XOR with the row index
+  // (chunk c of row r sits at position c ^ (r mod CPR)), so the MFMA operand
+  // reads of 16 consecutive rows hit distinct banks.  A partial tile repeats
+  // its last row, whose copies carry the exponent term -inf.
+  __shared__ __align__(16) __bf16 Ah[2][TILE * DP];
+  __shared__ __align__(16) __bf16 Al[2][NSPLIT == 3 ? TILE * DP : 8];
+  __shared__ __align__(16) float s_a[2][TILE];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int64_t s0 = (int64_t)split * split_len;
@@ -1338,75 +1348,166 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
   const float tq = (float)st.qd;
   float tqm = tq;    // LSE: exponent offset relative to the lazily moved reference
   double dl = 0.0;
+  // The target's exponent offset rides in the MFMA accumulator: the products
+  // start from tvec = tinit = offset / kc (the lane's 16 accumulator entries
+  // all belong to its target), so acc = sim + tinit and the exponent is
+  // fma(kc, med3(acc, tinit - 1, tinit + 1), a_s), the clamp of sim to
+  // [-1, 1] shifted: one addition per pair fewer.  The offset actually
+  // applied is kc * tinit, exact in FP64 (eoff); the LSE's dl and the sums'
+  // final rescale account for its difference from the requested offset.
+  // (kc == 0, an infinite regularisation, keeps the offset out of the
+  // accumulator; so does a tiny |kc|, whose tinit could overflow.)
+  const bool fold = fabsf(kc) >= 0x1p-40f;
+  float tinit = 0.0f, lo = -1.0f, hi = 1.0f;
+  double eoff = 0.0;
+  f32x16_t tvec;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) tvec[r] = 0.0f;
+  auto set_offset = [&](const float off) {
+    tinit = off / kc;
+    lo = tinit - 1.0f;
+    hi = tinit + 1.0f;
+    eoff = (double)kc * (double)tinit;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) tvec[r] = tinit;
+  };
+  if (fold) {
+    set_offset(tq);
+    if constexpr (KIND == KIND_LSE) dl = eoff - (double)tq;  // (reference m = 0)
+  }
 
-  for (int64_t base = s0; base < s1; base += TILE) {
+  // lse_lazy with the offset in the accumulator: the sub-tile's exponents v
+  // carry the effective offset eoff (S holds sum 2^(v_true + tq - m + dl),
+  // dl = eoff - (tq - m) exactly); when the sum leaves [2^-90, 2^100] the
+  // reference m moves as in lse_lazy, tinit follows it and the sub-tile is
+  // re-evaluated with the new offset.
+  auto lse_lazy_folded = [&](const float (&v)[16]) {
+    float loc = 0.0f;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) loc += exp2_fast(v[u]);
+    const double Sn = st.S + (double)loc;
+    if (loc < 0x1p100f && Sn > 0x1p-90) {
+      st.S = Sn;
+      return;
+    }
+    float tm = v[0];
+#pragma unroll
+    for (int u = 1; u < 16; ++u) tm = fmaxf(tm, v[u]);
+    int es = 0;
+    if (st.S > 0.0) (void)frexp(st.S, &es);
+    const float sl = st.S > 0.0 ? (float)es : -INFINITY;
+    const float top = fmaxf(tm, sl);
+    if (!(top > -INFINITY) || !(top < INFINITY)) {
+      st.S = Sn;
+      return;
+    }
+    const float c = ceilf(top);
+    const float mnew = st.m + c;
+    const double e_old = eoff;
+    set_offset(tq - mnew);
+    const double dnew = eoff - ((double)tq - (double)mnew);
+    const float dv = (float)(eoff - e_old);
+    st.S = ldexp(st.S, -(int)c) * exp2_small(dnew - dl);
+    loc = 0.0f;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) loc += exp2_fast(v[u] + dv);
+    st.S += (double)loc;
+    st.m = mnew;
+    dl = dnew;
+  };
+
+  // ---- staging ----
+  // Source term per source: the FP64 exponent term a_s (with the constant
+  // k / scale folded in) plus log2 of its weight, rounded once to FP32, so
+  // the exp-sum needs no weight: 2^(a'' + ...) = b_s 2^(a_s + ...) within
+  // 2^-24 |a''| in the exponent, far below the bf16 products' error.
+  double sa_raw = 0.0, sb_raw = 1.0;
+  // (global address space: these loads cannot alias the LDS the row copies
+  // in flight write, so they need not wait for them)
+  typedef const __attribute__((address_space(1))) double* gdp;
+  auto load_terms = [&](const int64_t base) {
     const int cnt = (int)min((int64_t)TILE, s1 - base);
-    const int nsub = (cnt + SUB - 1) / SUB;
-    __syncthreads();
-    // the source terms' loads go out first, with the row loads below
-    const bool term = tid < nsub * SUB;
     const bool real = tid < cnt;
-    const double sa_raw = (real && d.srcA != nullptr) ? d.srcA[base + tid] : 0.0;
-    const double sb_raw = (real && d.srcB != nullptr) ? d.srcB[base + tid] : 1.0;
-    // stage the tile's source rows, zero padded to whole MFMA tiles:
-    // nsub * SUB * DP bf16 = nsub * SUB * KS * 2 chunks of 16 B; every load of
-    // a thread is issued before its first LDS write
-    {
-      constexpr int CHUNKS = TILE * KS * 2;
-      constexpr int PER = (CHUNKS + kThreads - 1) / kThreads;
-      const int live = nsub * SUB * KS * 2;
-      bf16x8_t vh[PER], vl[NSPLIT == 3 ? PER : 1];
-#pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int e = tid + i * kThreads;
-        const int row = e / (KS * 2), c8 = (e % (KS * 2)) * 8;
-        if (e < live && row < cnt) {
-          vh[i] = *(const bf16x8_t*)(Sh + (base + row) * DP + c8);
-          if constexpr (NSPLIT == 3) vl[i] = *(const bf16x8_t*)(Sl + (base + row) * DP + c8);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            vh[i][q] = (__bf16)0.0f;
-            if constexpr (NSPLIT == 3) vl[i][q] = (__bf16)0.0f;
-          }
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int e = tid + i * kThreads;
-        if (e >= live) break;
-        const int row = e / (KS * 2), c8 = (e % (KS * 2)) * 8;
-        *(bf16x8_t*)(Ah + row * LDW + c8) = vh[i];
-        if constexpr (NSPLIT == 3) *(bf16x8_t*)(Al + row * LDW + c8) = vl[i];
-      }
-    }
-    if (term) {
-      float a = -INFINITY, wv = 0.0f;
-      if (real) {
-        // fold the constant k / scale into the FP64 source term before the split
+    sa_raw = (real && d.srcA != nullptr) ? *(gdp)(d.srcA + base + tid) : 0.0;
+    sb_raw = (real && d.srcB != nullptr) ? *(gdp)(d.srcB + base + tid) : 1.0;
+  };
+  auto store_terms = [&](const int64_t base, const int buf) {
+    const int cnt = (int)min((int64_t)TILE, s1 - base);
+    if (tid < TILE) {
+      float a = -INFINITY;
+      if (tid < cnt && sb_raw > 0.0) {
+        int e2 = 0;
+        const double mant = frexp(sb_raw, &e2);  // (no FP32 underflow of tiny weights)
         const double full = (d.srcA != nullptr ? d.alpha * sa_raw : 0.0) + d.k / d.fscale;
-        const double a2 = full * kLog2e;
-        const float hi = (float)a2;
-        const double lo = a2 - (double)hi;
-        a = hi;
-        wv = (float)(isfinite(lo) ? sb_raw * (1.0 + lo * kLn2) : sb_raw);
+        a = (float)(full * kLog2e + (double)e2 + (double)__log2f((float)mant));
       }
-      s_a[tid] = a;
-      s_w[tid] = wv;
+      s_a[buf][tid] = a;
     }
-    __syncthreads();
+  };
+  // issues the row loads of the tile at `base` into buffer `buf`: wave w
+  // fills the 1 KB blocks w, w + 4, ... (64 lanes x 16 bytes each); lane l
+  // of block i brings chunk c = (e mod CPR) ^ (row mod CPR) of row
+  // e / CPR, e = 64 i + l, to byte 16 e of the buffer
+  const unsigned wv_u = __builtin_amdgcn_readfirstlane(wave);
+  constexpr int BLOCKS = TILE * CPR / 64;                      // 1 KB blocks per tile
+  constexpr int NB = (BLOCKS + kThreads / 64 - 1) / (kThreads / 64);  // blocks per wave
+  unsigned roff[NB];  // byte offset of each block's chunk within a full tile
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const unsigned e = (4u * j + wv_u) * 64u + (unsigned)lane;
+    const unsigned row = e / CPR, c = (e % CPR) ^ (row % CPR);
+    roff[j] = row * (unsigned)(DP * 2) + c * 16u;
+  }
+  auto load_rows = [&](const int64_t base, const int buf) {
+    const int cnt = (int)min((int64_t)TILE, s1 - base);
+    const char* th = (const char*)(Sh + base * DP);
+    const char* tl = NSPLIT == 3 ? (const char*)(Sl + base * DP) : nullptr;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (BLOCKS % (kThreads / 64) != 0 && 4 * j + (int)wv_u >= BLOCKS) break;
+      unsigned off = roff[j];
+      if (cnt < TILE) {  // a partial tile repeats its last row
+        const unsigned row = off / (unsigned)(DP * 2);
+        if (row >= (unsigned)cnt) off += ((unsigned)cnt - 1u - row) * (unsigned)(DP * 2);
+      }
+      const int blk = (4 * j + (int)wv_u) * 512;  // bf16 elements per 1 KB block
+      __builtin_amdgcn_global_load_lds((const void*)(th + off),
+                                       (__attribute__((address_space(3))) void*)(&Ah[buf][blk]),
+                                       16, 0, 0);
+      if constexpr (NSPLIT == 3)
+        __builtin_amdgcn_global_load_lds((const void*)(tl + off),
+                                         (__attribute__((address_space(3))) void*)(&Al[buf][blk]),
+                                         16, 0, 0);
+    }
+  };
+
+  const int ntiles = s1 > s0 ? (int)((s1 - s0 + TILE - 1) / TILE) : 0;
+  if (ntiles > 0) {
+    load_rows(s0, 0);
+    load_terms(s0);
+    store_terms(s0, 0);
+  }
+  __syncthreads();
+  for (int T = 0; T < ntiles; ++T) {
+    const int64_t base = s0 + (int64_t)T * TILE;
+    const int buf = T & 1;
+    const bool more = T + 1 < ntiles;
+    if (more) {  // in flight during this tile's work
+      load_rows(base + TILE, buf ^ 1);
+      load_terms(base + TILE);
+    }
+    const int nsub = ((int)min((int64_t)TILE, s1 - base) + SUB - 1) / SUB;
     for (int sub = 0; sub < nsub; ++sub) {
       const int r0 = sub * SUB;
-      f32x16_t acc;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+      f32x16_t acc = tvec;
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
-        const int k0 = ks * 16 + 8 * (lane >> 5);
-        const bf16x8_t ah = *(const bf16x8_t*)(Ah + (r0 + jt) * LDW + k0);
+        const int row = r0 + jt;
+        const int at = row * DP + ((2 * ks + (lane >> 5)) ^ (row & (CPR - 1))) * 8;
+        const bf16x8_t ah = *(const bf16x8_t*)(&Ah[buf][at]);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[ks], acc, 0, 0, 0);
         if constexpr (NSPLIT == 3) {
-          const bf16x8_t al = *(const bf16x8_t*)(Al + (r0 + jt) * LDW + k0);
+          const bf16x8_t al = *(const bf16x8_t*)(&Al[buf][at]);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[ks], acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[ks], acc, 0, 0, 0);
         }
@@ -1417,44 +1518,70 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int i0 = r0 + 8 * q + 4 * (lane >> 5);
-        const float4 a4 = *(const float4*)(s_a + i0);
-        const float4 w4 = *(const float4*)(s_w + i0);
+        const float4 a4 = *(const float4*)(&s_a[buf][i0]);
         sa[4 * q] = a4.x; sa[4 * q + 1] = a4.y; sa[4 * q + 2] = a4.z; sa[4 * q + 3] = a4.w;
-        sw[4 * q] = w4.x; sw[4 * q + 1] = w4.y; sw[4 * q + 2] = w4.z; sw[4 * q + 3] = w4.w;
       }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) sw[r] = 1.0f;  // (the weights are in the terms)
       if constexpr (KIND == KIND_LSE) {
         float v[16];
+        if (fold) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
-          v[r] = fmaf(kc, sim, sa[r] + tqm);
+          for (int r = 0; r < 16; ++r) v[r] = fmaf(kc, __builtin_amdgcn_fmed3f(acc[r], lo, hi), sa[r]);
+          lse_lazy_folded(v);
+        } else {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
+            v[r] = fmaf(kc, sim, sa[r] + tqm);
+          }
+          lse_lazy<16>(st, v, sw, tq, tqm, dl);
         }
-        lse_lazy<16>(st, v, sw, tq, tqm, dl);
       } else if (d.out1 != nullptr) {
+        // C = (1 - sim) / scale = (1 + tinit - med3(acc, lo, hi)) / scale
+        const float c1 = (1.0f + tinit) * inv_scale;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           float v[8], wv[8], Cv[8], dz[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const int r = h * 8 + u;
-            const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
-            Cv[u] = (1.0f - sim) * inv_scale;
+            const float simc = __builtin_amdgcn_fmed3f(acc[r], lo, hi);
+            Cv[u] = fold ? fmaf(-simc, inv_scale, c1) : (1.0f - simc) * inv_scale;
             wv[u] = sw[r];
             dz[u] = 0.0f;
-            v[u] = fmaf(kc, sim, sa[r] + tq);
+            v[u] = fold ? fmaf(kc, simc, sa[r]) : fmaf(kc, simc, sa[r] + tq);
           }
           accumulate<1, KIND, 8, float>(st, v, wv, Cv, dz, dz, dz);
         }
       } else {  // no cost output: marginal only
         float v[16], dz[16];
+        if (fold) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
-          dz[r] = 0.0f;
-          v[r] = fmaf(kc, sim, sa[r] + tq);
+          for (int r = 0; r < 16; ++r) {
+            dz[r] = 0.0f;
+            v[r] = fmaf(kc, __builtin_amdgcn_fmed3f(acc[r], lo, hi), sa[r]);
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
+            dz[r] = 0.0f;
+            v[r] = fmaf(kc, sim, sa[r] + tq);
+          }
         }
         accumulate<1, KIND, 16, float, false>(st, v, sw, dz, dz, dz, dz);
       }
+    }
+    if (more) store_terms(base + TILE, buf ^ 1);
+    __syncthreads();
+  }
+  if constexpr (KIND != KIND_LSE) {
+    // back to the requested offset tq (as the split merge and epilogue use)
+    if (fold) {
+      const double f = exp2_small((double)tq - eoff);
+      st.S *= f;
+      st.Sc *= f;
     }
   }
   if constexpr (KIND == KIND_LSE) st.S *= exp2_small(-dl);
@@ -1474,8 +1601,10 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
   finish_pass<1, KIND, 1, float>(d, w, splits, split, ntp, tidx, fin);
 }
 
+// (one product: three waves per SIMD, 168 VGPRs each, three 33 KB workgroups
+// per CU at d = 128; the three-product split needs more registers and LDS)
 template <int KIND, int NSPLIT, int KS>
-__global__ __launch_bounds__(kThreads) void mfma_feature_pass(const PassDesc din, const Work w,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NSPLIT == 3 ? 2 : 3))) void mfma_feature_pass(const PassDesc din, const Work w,
                                                               const int splits,
                                                               const int64_t split_len) {
   const PassDesc d = resolve_desc(din);
@@ -1484,11 +1613,12 @@ __global__ __launch_bounds__(kThreads) void mfma_feature_pass(const PassDesc din
 }
 
 template <int KIND, int NSPLIT, int KS>
-__global__ __launch_bounds__(kThreads) void mfma_feature_pass_batch(const PassDesc* descs,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NSPLIT == 3 ? 2 : 3))) void mfma_feature_pass_batch(const PassDesc* descs,
                                                                     const BatchItem* items,
                                                                     const BatchInst* insts,
                                                                     const Work w) {
   const BatchItem it = items[blockIdx.x];
+  if (it.inst < 0) return;  // padding of an XCD's item sequence
   const BatchInst bi = insts[it.inst];
   const PassDesc d = resolve_desc(descs[it.inst]);
   const Work wi{w.partial + bi.pbase};

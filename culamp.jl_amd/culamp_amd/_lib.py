@@ -156,6 +156,7 @@ SIGNATURES = {
     "culamp_batch_destroy": (_I, [_P]),
     "culamp_batch_set_tuning": (_I, [_P, C.POINTER(culamp_batch_tuning)]),
     "culamp_batch_get_tuning": (_I, [_P, C.POINTER(culamp_batch_tuning)]),
+    "culamp_batch_time_pass": (_I, [_P, _P, _I, _D, _I, C.POINTER(_D), C.POINTER(C.c_int32)]),
     "culamp_batch_autotune": (_I, [_P, _P, _I, _D, _I, C.POINTER(culamp_batch_tuning),
                                    C.POINTER(_D)]),
     "culamp_batch_sinkhorn_solve": (_I, [_P, _P, C.POINTER(culamp_args), _I64, _D, _P, _P, _P, _P,

@@ -117,6 +117,15 @@ class Batch:
     def _split(self, v, offs):
         return [v[offs[k]:offs[k + 1]] for k in range(self.count)]
 
+    def time_pass(self, precision: str = "mixed", p: float = 2.0, reps: int = 20):
+        """Mean time (ms) of the row log-sum-exp pass over all instances at the
+        current tuning, from HIP events on the context's stream, and the kernel
+        launches per pass (2 when a split merge follows)."""
+        ms, nl = C.c_double(), C.c_int32()
+        check(load().culamp_batch_time_pass(self.ctx.handle, self.handle, _PRECISIONS[precision],
+                                            float(p), int(reps), C.byref(ms), C.byref(nl)))
+        return ms.value, nl.value
+
     def sinkhorn(self, args: EOTArgs, frequency: int = 100, p: float = 2.0, phi0=None, psi0=None):
         """Returns a list of (phi, psi, info) per instance."""
         N, M = int(self.off_n[-1]), int(self.off_m[-1])

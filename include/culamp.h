@@ -401,6 +401,15 @@ int culamp_batch_get_tuning(const culamp_batch *batch, culamp_batch_tuning *t);
 int culamp_batch_autotune(culamp_ctx *ctx, culamp_batch *batch, int precision,
                           double p, int reps, culamp_batch_tuning *chosen,
                           double *ms_per_pass);
+/* Measurement (no reference counterpart): times the batch's row
+ * log-sum-exp pass over all instances at its current tuning, `reps` launches
+ * between two HIP events on the context's stream after one warm-up launch;
+ * *ms_per_pass gets the mean and *launches_per_pass (may be NULL) the kernel
+ * launches per pass (2 when some instance's source splits need the merge
+ * kernel, else 1). */
+int culamp_batch_time_pass(culamp_ctx *ctx, culamp_batch *batch, int precision,
+                           double p, int reps, double *ms_per_pass,
+                           int32_t *launches_per_pass);
 /* Per instance: sinkhorn_color_transfer (points, SinkhornKernel.jl:477-559)
  * or sinkhorn_cellsim (features, SinkhornCellSim.jl:240-310).  phi0/psi0
  * (optional) and phi/psi are the instances' vectors concatenated in order
