@@ -920,10 +920,21 @@ int or_lamp_cellsim_solve(const double *data, int64_t n, int64_t d,
                           int64_t frequency, or_log_fn logfn, void *user,
                           double *nu_out, double *phi_out, double *psi_out,
                           or_result *out) {
-  if (n <= 0 || d <= 0 || frequency <= 0) return 1;
-  const int64_t m = n;
+  return or_lamp_cellsim2_solve(data, n, data, n, d, scale, metric, r, c, args, frequency,
+                                logfn, user, nu_out, phi_out, psi_out, out);
+}
+
+/* The same loop with the cost between row i of data1 (n x d) and row j of
+ * data2 (m x d); the reference calls it with one matrix (data1 == data2). */
+int or_lamp_cellsim2_solve(const double *data1, int64_t n, const double *data2,
+                           int64_t m, int64_t d, double scale, int metric,
+                           const double *r, const double *c, const or_args *args,
+                           int64_t frequency, or_log_fn logfn, void *user,
+                           double *nu_out, double *phi_out, double *psi_out,
+                           or_result *out) {
+  if (n <= 0 || m <= 0 || d <= 0 || frequency <= 0) return 1;
   double *C = malloc(sizeof(double) * (size_t)n * (size_t)m);
-  or_feature_cost_matrix(metric, data, n, data, m, d, scale, C);
+  or_feature_cost_matrix(metric, data1, n, data2, m, d, scale, C);
   double *theta = calloc((size_t)m, sizeof(double)), *nu = calloc((size_t)m, sizeof(double));
   double *thetab = calloc((size_t)m, sizeof(double)), *nub = calloc((size_t)m, sizeof(double));
   double *gam = calloc((size_t)m, sizeof(double)), *res = calloc((size_t)m, sizeof(double));

@@ -161,6 +161,12 @@ int or_lamp_cellsim_solve(const double *data, int64_t n, int64_t d,
                           const double *c, const or_args *args,
                           int64_t frequency, or_log_fn logfn, void *user,
                           double *nu, double *phi, double *psi, or_result *res);
+int or_lamp_cellsim2_solve(const double *data1, int64_t n, const double *data2,
+                           int64_t m, int64_t d, double scale, int metric,
+                           const double *r, const double *c, const or_args *args,
+                           int64_t frequency, or_log_fn logfn, void *user,
+                           double *nu_out, double *phi_out, double *psi_out,
+                           or_result *out);
 /* sinkhorn_cellsim (src/solvers/SinkhornCellSim.jl:240-310): within each
  * half-step the cost is evaluated among the rows of that call's first data
  * argument (SinkhornCellSim.jl:43, 193); the reference calls it with

@@ -273,6 +273,8 @@ _SIG_F = {
     "or_feature_cost_matrix": (None, [_I, _P, _I64, _P, _I64, _I64, _D, _P]),
     "or_lamp_cellsim_solve": (_I, [_P, _I64, _I64, _D, _I, _P, _P, C.POINTER(or_args), _I64, LOG_FN,
                                    _P, _P, _P, _P, C.POINTER(or_result)]),
+    "or_lamp_cellsim2_solve": (_I, [_P, _I64, _P, _I64, _I64, _D, _I, _P, _P, C.POINTER(or_args),
+                                    _I64, LOG_FN, _P, _P, _P, _P, C.POINTER(or_result)]),
     "or_sinkhorn_cellsim_solve": (_I, [_P, _I64, _P, _I64, _I64, _D, _I, _P, _P, C.POINTER(or_args),
                                        _I64, _P, _P, LOG_FN, _P, _P, _P, C.POINTER(or_result)]),
     "or_annealed_sinkhorn_cellsim_solve": (_I, [_P, _I64, _P, _I64, _I64, _D, _I, _P, _P,
@@ -328,6 +330,23 @@ def lamp_cellsim_solve(data, r, c, metric="cosine", scale=None, frequency=100, *
     rc = _flib().or_lamp_cellsim_solve(_fp(data), n, d, float(scale), METRICS[metric], _p(r), _p(c),
                                        C.byref(a), int(frequency), lg.cb, None, _p(nu), _p(phi),
                                        _p(psi), C.byref(res))
+    assert rc == 0
+    return dict(nu=nu, phi=phi, psi=psi, log=lg.lines, **res.as_dict())
+
+
+def lamp_cellsim2_solve(data1, data2, r, c, metric="cosine", scale=None, frequency=100, **args):
+    """extragradient_cellsim with the cost between rows of data1 and rows of
+    data2 (lamp_cellsim_solve is the data1 == data2 case)."""
+    data1, data2 = _colmajor(data1), _colmajor(data2)
+    r, c = _f(r), _f(c)
+    n, d = data1.shape
+    m = data2.shape[0]
+    scale = SCALES[metric] if scale is None else scale
+    nu, phi, psi = np.empty(m), np.empty(n), np.empty(m)
+    a, res, lg = make_args(**args), or_result(), _Log()
+    rc = _flib().or_lamp_cellsim2_solve(_fp(data1), n, _fp(data2), m, d, float(scale),
+                                        METRICS[metric], _p(r), _p(c), C.byref(a), int(frequency),
+                                        lg.cb, None, _p(nu), _p(phi), _p(psi), C.byref(res))
     assert rc == 0
     return dict(nu=nu, phi=phi, psi=psi, log=lg.lines, **res.as_dict())
 

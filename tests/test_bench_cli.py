@@ -26,3 +26,14 @@ def test_help_lists_profiling_modes():
     for flag in ("--kernel-only", "--solve-only", "--dense-only", "--clouds-only",
                  "--batch-kernel-only", "--steps", "--warmup", "--gpus"):
         assert flag in p.stdout, flag
+
+
+def test_more_than_one_gpu_is_refused_in_profiling_modes():
+    """The refusal comes before the profiling modes are dispatched, so none
+    of them silently runs on one GPU."""
+    for mode in ("--kernel-only", "--solve-only", "--dense-only", "--clouds-only",
+                 "--batch-kernel-only"):
+        p = _run("--gpus", "4", mode)
+        assert p.returncode == 2, mode
+        assert "one GPU" in p.stderr, mode
+        assert p.stdout == "", mode

@@ -138,6 +138,21 @@ def test_batch_cellsim_fp64(gpu_ctx, oracle, metric):
             assert np.allclose(phi, o["phi"], rtol=1e-8, atol=1e-8), (eta, k)
             assert abs(info["objective"] - o["objective"]) <= 1e-9 * abs(o["objective"]) + 1e-14
     b.close()
+    # LAMP cellsim between two point sets (the batched cell-similarity
+    # configuration's instances), against the oracle's two-matrix loop
+    inst = []
+    for n, m in ((60, 45), (33, 70)):
+        inst.append((cells(rng, n, 16), cells(rng, m, 16), rng.dirichlet(np.ones(n)),
+                     rng.dirichlet(np.ones(m))))
+    b = culamp_amd.Batch(inst, gpu_ctx, metric=metric, scale=scale)
+    kw = dict(eta_p=5e-2, epsilon=6e-6, B=1.1, itermax=20000, verbose=False, stop="infeas")
+    out = b.lamp(culamp_amd.EOTArgs(**kw), 10)
+    for k, ((x, y, r, c), (nu, phi, psi, info)) in enumerate(zip(inst, out)):
+        o = oracle.lamp_cellsim2_solve(x, y, r, c, metric, scale, frequency=10, **kw)
+        assert info["iterations"] == o["iterations"] > 0, (k, info["iterations"], o["iterations"])
+        assert np.allclose(nu, o["nu"], atol=1e-8), k
+        assert abs(info["objective"] - o["objective"]) <= 1e-9 * abs(o["objective"]) + 1e-14
+    b.close()
 
 
 def unit_cells(rng, n, d=128, density=0.3):
