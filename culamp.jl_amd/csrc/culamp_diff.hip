@@ -147,6 +147,10 @@ int forward(culamp_ctx* ctx, const Term& tm, double eps, const culamp_divergence
   int converged = 0;
   double resid = NAN;
   int prec = o.precision;
+  // after a switch to FP64 (at tol, or on a stall) the solve continues to
+  // the smaller of tol and the polish tolerance
+  const double fp64_target = o.polish_tol > 0.0 ? std::fmin(o.tol, o.polish_tol) : o.tol;
+  PrecisionGuard guard(o.precision, ctx->fp64_fallback != 0);
   auto residual = [&]() -> int {
     TRY(run_pass(ctx, d3, prec, KIND_SUM));
     ReduceSpec rs{};
@@ -160,17 +164,23 @@ int forward(culamp_ctx* ctx, const Term& tm, double eps, const culamp_divergence
     if (o.unroll > 0 && t % S.every == 0) TRY(store_ck(ctx, tm, S, t));
     if (t % o.frequency == 0) {
       TRY(residual());
-      if (resid <= (prec == o.precision ? o.tol : o.polish_tol)) {
+      if (resid <= (prec == o.precision ? o.tol : fp64_target)) {
         if (prec != CULAMP_FP64 && o.polish_tol > 0.0) {
           // continue in FP64 from the mixed solution to the polish tolerance
           S.t_mixed = t;
           prec = CULAMP_FP64;
           TRY(residual());
-          if (resid > o.polish_tol) continue;
+          if (resid > fp64_target) continue;
         }
         converged = 1;
         break;
       }
+      if (guard.observe(resid, o.tol, t)) {
+        // the mixed residual stalled above tol (its FP32 floor): FP64 from here
+        S.t_mixed = t;
+        prec = CULAMP_FP64;
+      }
+      if (now_s() - t0 > o.tmax) break;
     }
   }
   if (!converged) TRY(residual());
@@ -199,6 +209,7 @@ int forward(culamp_ctx* ctx, const Term& tm, double eps, const culamp_divergence
     out->winf = NAN;
     out->eta_t = eps;
     out->converged = converged;
+    out->fp64_from = S.t_mixed != INT64_MAX ? S.t_mixed : 0;
     out->nonfinite = !(std::isfinite(value) && std::isfinite(resid));
     out->elapsed_s = now_s() - t0;
     out->pairs = ctx->pairs - pairs0;
@@ -375,6 +386,10 @@ int sym_forward(culamp_ctx* ctx, const double* x, const double* a, int64_t n, do
   int converged = 0;
   double resid = NAN;
   int prec = o.precision;
+  // after a switch to FP64 (at tol, or on a stall) the solve continues to
+  // the smaller of tol and the polish tolerance
+  const double fp64_target = o.polish_tol > 0.0 ? std::fmin(o.tol, o.polish_tol) : o.tol;
+  PrecisionGuard guard(o.precision, ctx->fp64_fallback != 0);
   auto residual = [&]() -> int {
     TRY(run_pass(ctx, d3, prec, KIND_SUM));
     ReduceSpec rs{};
@@ -387,17 +402,23 @@ int sym_forward(culamp_ctx* ctx, const double* x, const double* a, int64_t n, do
     if (o.unroll > 0 && t % S.every == 0) TRY(sym_store(ctx, n, S, t));
     if (t % o.frequency == 0) {
       TRY(residual());
-      if (resid <= (prec == o.precision ? o.tol : o.polish_tol)) {
+      if (resid <= (prec == o.precision ? o.tol : fp64_target)) {
         if (prec != CULAMP_FP64 && o.polish_tol > 0.0) {
           // continue in FP64 from the mixed solution to the polish tolerance
           S.t_mixed = t;
           prec = CULAMP_FP64;
           TRY(residual());
-          if (resid > o.polish_tol) continue;
+          if (resid > fp64_target) continue;
         }
         converged = 1;
         break;
       }
+      if (guard.observe(resid, o.tol, t)) {
+        // the mixed residual stalled above tol (its FP32 floor): FP64 from here
+        S.t_mixed = t;
+        prec = CULAMP_FP64;
+      }
+      if (now_s() - t0 > o.tmax) break;
     }
   }
   if (!converged) TRY(residual());
@@ -422,6 +443,7 @@ int sym_forward(culamp_ctx* ctx, const double* x, const double* a, int64_t n, do
     out->winf = NAN;
     out->eta_t = eps;
     out->converged = converged;
+    out->fp64_from = S.t_mixed != INT64_MAX ? S.t_mixed : 0;
     out->nonfinite = !(std::isfinite(value) && std::isfinite(resid));
     out->elapsed_s = now_s() - t0;
     out->pairs = ctx->pairs - pairs0;
@@ -608,6 +630,7 @@ void culamp_divergence_opts_default(culamp_divergence_opts* o) {
   o->checkpoint = 16;
   o->precision = CULAMP_MIXED;
   o->polish_tol = 1e-9;
+  o->tmax = INFINITY;
 }
 
 int culamp_sinkhorn_divergence(culamp_ctx* ctx, const double* x, int64_t n, const double* a,

@@ -77,7 +77,7 @@ class culamp_divergence_opts(C.Structure):
     _fields_ = [("eps", C.c_double), ("eta_p", C.c_double), ("tol", C.c_double),
                 ("polish_tol", C.c_double), ("itermax", C.c_int64), ("frequency", C.c_int64),
                 ("unroll", C.c_int32), ("checkpoint", C.c_int32), ("precision", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("reserved", C.c_int32), ("tmax", C.c_double)]
 
 
 LOG_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)

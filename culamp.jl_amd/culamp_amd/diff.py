@@ -12,6 +12,7 @@ eta_p * max_ij C_ij of the (x, y) pair, held fixed under differentiation).
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import numpy as np
 
@@ -22,7 +23,7 @@ from .solvers import default_context
 def divergence_opts(eps: float | None = None, eta_p: float = 1e-2, tol: float = 1e-6,
                     itermax: int = 100000, frequency: int = 10, unroll: int = 64,
                     checkpoint: int = 16, precision: str = "mixed",
-                    polish_tol: float = 1e-9) -> culamp_divergence_opts:
+                    polish_tol: float = 1e-9, tmax: float = math.inf) -> culamp_divergence_opts:
     o = culamp_divergence_opts()
     load().culamp_divergence_opts_default(C.byref(o))
     o.eps = 0.0 if eps is None else float(eps)
@@ -34,6 +35,7 @@ def divergence_opts(eps: float | None = None, eta_p: float = 1e-2, tol: float = 
     o.checkpoint = int(checkpoint)
     o.precision = {"fp64": FP64, "mixed": MIXED}[precision]
     o.polish_tol = float(polish_tol)
+    o.tmax = float(tmax)
     return o
 
 

@@ -111,7 +111,10 @@ typedef struct {
   int64_t fp64_from;   /* reduced-precision solves: the iteration from which
                         * the passes ran in FP64 because the infeasibility
                         * stopped improving above the stop tolerance (see
-                        * culamp_ctx_set_fp64_fallback); 0 if they never did */
+                        * culamp_ctx_set_fp64_fallback) or, for a Sinkhorn
+                        * divergence term, the iteration after which the
+                        * forward ran in FP64 (polish or stall); 0 if they
+                        * never did */
 } culamp_result;
 
 /* Log sink: called synchronously on the calling thread with one CSV line in
@@ -455,8 +458,12 @@ typedef struct {
   int64_t frequency;  /* infeasibility tested every `frequency` iterations */
   int32_t unroll;     /* iterations back-propagated at the end of each solve (0: envelope only) */
   int32_t checkpoint; /* checkpoint interval inside the unrolled window (>= 1) */
-  int32_t precision;  /* forward passes: CULAMP_FP64 or CULAMP_MIXED; adjoint passes are FP64 */
+  int32_t precision;  /* forward passes: CULAMP_FP64 or CULAMP_MIXED; adjoint passes are FP64
+                         (a mixed forward whose infeasibility stalls above tol continues in
+                         FP64, see culamp_ctx_set_fp64_fallback) */
   int32_t reserved;
+  double tmax;        /* each forward solve stops (not converged) after tmax seconds,
+                         tested at every infeasibility check (default: no limit) */
 } culamp_divergence_opts;
 
 void culamp_divergence_opts_default(culamp_divergence_opts *opts);

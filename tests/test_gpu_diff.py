@@ -106,6 +106,11 @@ def test_divergence_value_vs_dense(gpu_ctx):
         aa = dense_ot(x, a, x, a, eps, sym=True)
         bb = dense_ot(y, b, y, b, eps, sym=True)
         for (dual, primal), t in zip((ab, aa, bb), terms):
+            # (tol 1e-12 is below the mixed forward's FP32 floor: its stall
+            # guard hands the solve to FP64, which reaches it)
+            assert t["converged"] == 1, (prec, t)
+            if prec == "mixed":
+                assert t["fp64_from"] > 0
             assert abs(dual - primal) < 1e-10 * abs(primal) + 1e-13  # dual value == primal value
             assert abs(t["objective"] - dual) < tol * abs(dual) + 1e-12, (prec, t["objective"], dual)
         want = ab[0] - 0.5 * aa[0] - 0.5 * bb[0]
