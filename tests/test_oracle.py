@@ -301,3 +301,108 @@ def test_lamp_infeas_stop_skips_product_plan_check(oracle):
     # the infeasibility stop does, after real iterations
     z = oracle.lamp_solve(x, y, r, c, verbose=False, stop="infeas", **{**kw, "eta_p": 0.0})
     assert z["converged"] == 1 and z["iterations"] > 0 and z["objective"] < 0.6 * prod
+
+
+def _feature_instance(seed, n, d, metric):
+    rng = np.random.default_rng(seed)
+    X = np.abs(rng.normal(size=(n, d))) * (rng.uniform(size=(n, d)) < 0.5) + 1e-3
+    r, c = rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(n))
+    return X, r, c
+
+
+def test_cellsim_lamp_converges_to_entropic_plan_at_2winf_eta(oracle):
+    """extragradient_cellsim keeps eta = eta_p (MirrorProxCellSim.jl:118) on
+    the cost C / (2 W_inf) (281-282), so at convergence (eta_t -> eta_p,
+    infeasibility 1e-11) its plan r_i softmax_j(-(C_ij / (2 W_inf) + nu_j) /
+    eta_t) is the entropic OT plan of the raw cost at regularisation
+    2 W_inf eta_p, which an independent dense log-domain Sinkhorn gives.  A
+    restatement with the point-cloud convention eta = eta_p / (2 W_inf) would
+    converge to the plan at regularisation eta_p instead (1e-3 away here)."""
+    from scipy.special import logsumexp
+    X, r, c = _feature_instance(7, 30, 10, "cosine")
+    C = oracle.feature_cost_matrix(X, X, "cosine", 2.0)
+    W = C.max()
+    for eta_p in (0.05, 0.2):
+        o = oracle.lamp_cellsim_solve(X, r, c, "cosine", 2.0, frequency=10, eta_p=eta_p,
+                                      epsilon=6e-11, B=1.1, itermax=400000, verbose=False,
+                                      stop="infeas")
+        assert o["converged"] == 1 and o["winf"] == W
+        assert abs(o["eta_t"] - eta_p) < 1e-8 * eta_p
+        L = -(C / (2 * W) + o["nu"][None, :]) / o["eta_t"]
+        P = r[:, None] * np.exp(L - logsumexp(L, axis=1, keepdims=True))
+        ref = oracle.sinkhorn_log_dense(r, c, C, frequency=10, eta_p=2 * W * eta_p, epsilon=6e-14,
+                                        itermax=200000, verbose=False)["P"]
+        assert np.abs(P - ref).max() < 1e-8 * ref.max()
+        other = oracle.sinkhorn_log_dense(r, c, C, frequency=10, eta_p=eta_p, epsilon=6e-14,
+                                          itermax=200000, verbose=False)["P"]
+        assert np.abs(P - other).max() > 1e-4 * other.max()
+        # psi = -2 W_inf nu / eta_p (MirrorProxCellSim.jl:184)
+        assert np.allclose(o["psi"], -2 * W * o["nu"] / eta_p, rtol=0, atol=1e-12 * W / eta_p)
+
+
+def test_cellsim_lamp_eta0_reaches_lp_with_weak_duality(oracle):
+    """eta_p = 0: the cellsim LAMP's cost tends to the LP optimum (HiGHS), and
+    at every logged check its dual (the averaged-gamma min-plus form,
+    MirrorProxCellSim.jl:140, 157-159, 179) and primal bracket it."""
+    from scipy.optimize import linprog
+    rng = np.random.default_rng(8)
+    n = 12
+    X = np.abs(rng.normal(size=(n, 6))) + 0.05
+    r, c = rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(n))
+    C = oracle.feature_cost_matrix(X, X, "l1", 2.0)
+    A = np.zeros((2 * n, n * n))
+    for i in range(n):
+        A[i, i * n:(i + 1) * n] = 1
+        A[n + i, i::n] = 1
+    lp = linprog(C.ravel(), A_eq=A, b_eq=np.concatenate([r, c]), bounds=(0, None),
+                 method="highs").fun
+    o = oracle.lamp_cellsim_solve(X, r, c, "l1", 2.0, frequency=10, eta_p=0.0, epsilon=6e-4,
+                                  B=1.1, itermax=400000, verbose=True)
+    assert o["converged"] == 1
+    assert abs(o["objective"] - lp) < 1e-6 * lp
+    rows = [l.split(",") for l in o["log"][1:]]
+    primal = np.array([float(q[4]) for q in rows])
+    dual = np.array([float(q[5]) for q in rows])
+    assert np.all(dual <= lp + 1e-12) and np.all(primal >= lp - 1e-12)
+    assert primal[-1] - dual[-1] < 1e-4
+
+
+@pytest.mark.parametrize("kind", ["points", "cellsim"])
+def test_annealed_sinkhorn_ends_at_the_eps_fixed_point(oracle, kind):
+    """Annealed Sinkhorn (AnnealedSinkhorn.jl:65-177, AnnealedSinkhornCellSim.jl:
+    1-63) ends with a Sinkhorn stage at eta = epsilon stopped at infeasibility
+    epsilon / 6: its plan is within a few times that residual (L1) of the
+    fully converged Sinkhorn plan at epsilon, and far from the one at
+    2 epsilon (so a schedule that stopped a stage early would fail)."""
+    rng = np.random.default_rng(8)
+    eps = 1e-2
+    if kind == "points":
+        x, y = rng.uniform(size=(40, 3)), rng.uniform(size=(35, 3))
+        a, b = rng.dirichlet(np.ones(40)), rng.dirichlet(np.ones(35))
+        C = ((x[:, None] - y[None]) ** 2).sum(-1)
+        an = oracle.annealed_sinkhorn_solve(x, y, a, b, frequency=5, p=2.0, eta_p=0.5, epsilon=eps,
+                                            anneal_mult=0.5, verbose=False)
+        direct = lambda e: oracle.sinkhorn_solve(x, y, a, b, frequency=5, p=2.0, eta_p=e,  # noqa: E731
+                                                 epsilon=6e-14, itermax=400000, verbose=False)
+    else:
+        X = np.abs(rng.normal(size=(12, 6))) + 0.05
+        a, b = rng.dirichlet(np.ones(12)), rng.dirichlet(np.ones(12))
+        C = oracle.feature_cost_matrix(X, X, "l1", 2.0)
+        an = oracle.annealed_sinkhorn_cellsim_solve(X, X, a, b, "l1", 2.0, frequency=5, eta_p=0.5,
+                                                    epsilon=eps, anneal_mult=0.5, verbose=False)
+        direct = lambda e: oracle.sinkhorn_cellsim_solve(X, X, a, b, "l1", 2.0, frequency=5,  # noqa: E731
+                                                         eta_p=e, epsilon=6e-14, itermax=400000,
+                                                         verbose=False)
+    W = C.max()
+
+    def plan(e, phi, psi):
+        return np.exp(-C / (e * W) + phi[:, None] + psi[None, :])
+
+    assert an["residual"] <= eps / 6
+    pa = plan(eps, an["phi"], an["psi"])
+    d1 = direct(eps)
+    d2 = direct(2 * eps)
+    l1 = np.abs(pa - plan(eps, d1["phi"], d1["psi"])).sum()
+    l2 = np.abs(pa - plan(2 * eps, d2["phi"], d2["psi"])).sum()
+    assert l1 <= 4 * an["residual"], (l1, an["residual"])
+    assert l1 < 0.1 * l2, (l1, l2)
