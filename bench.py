@@ -636,6 +636,123 @@ def bench_grid64(ctx, a, log) -> dict:
     return out
 
 
+# The reference's Table-1 medians of warp_logsumexp_spp_ct_fused! (ms; FP64,
+# random points in [0,1]^3, theta in [-1, 1], reg = 1e-4, st = 0.5, W_inf = 1,
+# p = 2), data_archive/data/kernel_timings_output.csv:3, 8, 13, 18 (GPU model
+# unstated)
+TABLE1_REFERENCE_MS = {1024: 0.120141, 4096: 1.312046, 16384: 17.260767, 65536: 252.7594105}
+
+
+def _graph_time_us(ctx, fn, reps=20, launches=5):
+    """Microseconds per call of `fn`, replayed from a captured graph."""
+    fn()
+    fn()
+    g = ctx.capture(lambda: [fn() for _ in range(reps)])
+    g.launch()
+    ctx.timer_start()
+    for _ in range(launches):
+        g.launch()
+    ms = ctx.timer_stop() / (launches * reps)
+    g.close()
+    return ms * 1e3
+
+
+def bench_table1(ctx, a, log) -> dict:
+    """The reference's Table-1 kernel benchmark (experiments/table_1_kernel_bench.jl:
+    7-81) on the fused row LSE: N = M random points in [0,1]^3, theta uniform
+    in [-1, 1], reg = 1e-4, st = 0.5, W_inf = 1, p = 2; our culamp_lse_rows in
+    FP64 and in mixed precision against the reference's published medians."""
+    import numpy as np
+    import culamp_amd
+    from culamp_amd import FP64, MIXED
+    from culamp_amd import kernels as K
+    out = []
+    for n in a.table1_sizes:
+        rng = np.random.default_rng(700 + n)
+        x, y = rng.uniform(size=(n, 3)), rng.uniform(size=(n, 3))
+        th = 2 * rng.uniform(size=n) - 1
+        dev = lambda v: culamp_amd.DeviceArray.from_host(ctx, v)  # noqa: E731
+        dx, dy, dth = dev(x), dev(y), dev(th)
+        o64, o32 = culamp_amd.DeviceArray(ctx, n), culamp_amd.DeviceArray(ctx, n)
+        rec = {"N": n}
+        for name, prec, o in (("fp64", FP64, o64), ("mixed", MIXED, o32)):
+            us = _graph_time_us(ctx, lambda: K.warp_logsumexp_spp_ct_fused(
+                ctx, o, dx, dy, dth, 1e-4, 0.5, 1.0, 2.0, prec), reps=max(1, 20 * 1024 // n))
+            rec[name] = {"ms": us / 1e3, "Gpairs_per_s": n * n / (us * 1e-6) / 1e9}
+        ref = TABLE1_REFERENCE_MS.get(n)
+        v64, v32 = o64.get(), o32.get()
+        rec["mixed_max_abs_diff_vs_fp64"] = float(np.max(np.abs(v64 - v32)))
+        rec["mixed_max_rel_diff_vs_fp64"] = float(np.max(np.abs(v64 - v32) / np.abs(v64)))
+        if ref:
+            rec["reference_fused_warp_ms"] = ref
+            rec["fp64_speedup_vs_reference"] = ref / rec["fp64"]["ms"]
+            rec["mixed_speedup_vs_reference"] = ref / rec["mixed"]["ms"]
+        log(f"table-1 N={n}: fp64 {rec['fp64']['ms']:.4f} ms, mixed {rec['mixed']['ms']:.4f} ms"
+            + (f" (reference {ref} ms)" if ref else ""))
+        out.append(rec)
+    return {"workload": "Table-1 fused row LSE (experiments/table_1_kernel_bench.jl): N = M random "
+                        "points in [0,1]^3, theta in [-1, 1], reg = 1e-4, st = 0.5, W_inf = 1, p = 2",
+            "reference_source": "data_archive/data/kernel_timings_output.csv:3,8,13,18 "
+                                "(fused_warp medians, FP64, GPU model unstated)",
+            "sizes": out}
+
+
+def bench_cpu_reference(ctx, a, log) -> dict:
+    """The reference's CPU path on the 32 x 32 uniform grids (n = m = 1024,
+    p = 2, W normalised by its largest entry as run_dot does,
+    src/CmdLineUtils.jl:195-198): the dense log-domain sinkhorn_log
+    (src/solvers/Sinkhorn.jl:8-84) to infeasibility 1e-6 and the CPU LAMP
+    (src/solvers/MirrorProx.jl:506-575, whose gap test never closes at
+    eta_p > 0, so it is timed over a fixed number of iterations), both as the
+    oracle's FP64 restatements at one thread and at the host-thread share;
+    beside them the GPU's on-the-fly FP64 and mixed Sinkhorn on the same
+    problem (the same fixed point)."""
+    import culamp_amd
+    from culamp_amd import workloads
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    x, y, r, c = workloads.uniform_grid_pair(32)
+    W = oracle.grid_cost_matrix(32, 32, 2.0)
+    W = W / W.max()
+    nproc = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
+    out = {"workload": "32x32 grid histograms (n = m = 1024), p = 2, eta_p = 1e-2, epsilon = 6e-6 "
+                       "(stop at L1 infeasibility 1e-6), FP64", "nproc": nproc}
+    for threads in (1, share):
+        oracle.set_threads(threads)
+        t0 = time.perf_counter()
+        s = oracle.sinkhorn_log_dense(r, c, W, frequency=10, eta_p=1e-2, epsilon=6e-6,
+                                      itermax=100000, verbose=False)
+        ds = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        lp = oracle.cpu_lamp_solve(r, c, W, frequency=10, eta_p=1e-2, epsilon=6e-6, B=1.1,
+                                   itermax=a.cpu_lamp_iters, tmax=a.cpu_seconds, verbose=False)
+        dl = time.perf_counter() - t0
+        out[f"threads_{threads}"] = {
+            "sinkhorn_log": {"seconds": ds, "iterations": s["iterations"],
+                             "infeasibility": s["residual"], "converged": s["converged"]},
+            "lamp": {"seconds": dl, "iterations": lp["iterations"],
+                     "iterations_per_s": lp["iterations"] / dl}}
+        log(f"cpu reference path ({threads} threads): sinkhorn_log {ds:.3f}s ({s['iterations']} it), "
+            f"cpu LAMP {lp['iterations'] / dl:.1f} it/s")
+    pb = culamp_amd.Problem(x, y, r, c, ctx)
+    for prec in ("fp64", "mixed"):
+        args = culamp_amd.EOTArgs(eta_p=1e-2, epsilon=6e-6, itermax=100000, verbose=False,
+                                  precision=prec)
+        pb.sinkhorn(args, 10, 2.0, log=None, assignments=False)
+        t0 = time.perf_counter()
+        info = pb.sinkhorn(args, 10, 2.0, log=None, assignments=False)[-1]
+        dg = time.perf_counter() - t0
+        out[f"gpu_sinkhorn_{prec}"] = {"seconds": dg, "iterations": info["iterations"],
+                                       "infeasibility": info["residual"]}
+        out[f"gpu_{prec}_speedup_vs_cpu_{share}_threads"] = out[f"threads_{share}"]["sinkhorn_log"][
+            "seconds"] / dg
+    pb.close()
+    log(f"gpu on-the-fly sinkhorn: fp64 {out['gpu_sinkhorn_fp64']['seconds'] * 1e3:.2f} ms, "
+        f"mixed {out['gpu_sinkhorn_mixed']['seconds'] * 1e3:.2f} ms")
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -647,6 +764,9 @@ def main() -> None:
     ap.add_argument("--frequency", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-lamp-iters", type=int, default=2000)
+    ap.add_argument("--table1-sizes", type=int, nargs="*", default=[1024, 4096, 16384, 65536])
+    ap.add_argument("--no-table1", action="store_true")
     ap.add_argument("--kernel-launches", type=int, default=200)
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-precision", default="bf16")
@@ -708,6 +828,8 @@ def main() -> None:
     extras = batched_extras(ctx, b, insts, head, a, log)
     b.close()
     cpu = None if a.no_cpu else cpu_baseline_batched(insts, head, a, log)
+    cpu_ref = None if a.no_cpu else bench_cpu_reference(ctx, a, log)
+    table1 = None if a.no_table1 else bench_table1(ctx, a, log)
     grid = None if a.no_grid else bench_grid64(ctx, a, log)
     clouds = None if a.no_clouds else bench_clouds(ctx, a, log)
     divergence = None if a.no_diff else bench_divergence(ctx, a, log)
@@ -733,7 +855,8 @@ def main() -> None:
         "roofline": roofline, "cpu_baseline": cpu,
         "batched": extras,
     }
-    for k, v in (("grid64", grid), ("clouds", clouds), ("divergence", divergence),
+    for k, v in (("cpu_reference_path", cpu_ref), ("table1", table1), ("grid64", grid),
+                 ("clouds", clouds), ("divergence", divergence),
                  ("cindex", cidx), ("dense_comparator", dense)):
         if v is not None:
             rec[k] = v

@@ -573,7 +573,9 @@ int or_annealed_sinkhorn_solve(const double *x, int64_t n, const double *y,
  * n x m row-major matrix K with additive row/column terms. */
 static void lse_cols_dense(const double *K, int64_t n, int64_t m,
                            const double *rowterm, double *out) {
-  /* out_j = LSE_i (K_ij + rowterm_i)  (dims = 1) */
+  /* out_j = LSE_i (K_ij + rowterm_i)  (dims = 1); columns in parallel, each
+   * summed over i in order */
+  OMP_FOR
   for (int64_t j = 0; j < m; ++j) {
     double mx = -INFINITY;
     for (int64_t i = 0; i < n; ++i) {
@@ -1183,5 +1185,206 @@ int or_annealed_sinkhorn_cellsim_solve(const double *data1, int64_t n,
     out->eta_t = eta;
   }
   free(C); free(phi); free(psi); free(phiprev); free(psiprev); free(res); free(cost);
+  return 0;
+}
+
+/* ---- the reference's CPU path: LAMP on a dense cost matrix ----------------
+ * LAMP(r, c, W::AbstractMatrix, args, frequency) (src/solvers/MirrorProx.jl:
+ * 506-575), FP64, W n x n row-major (W[i n + j], the reference's W[i, j]).
+ * Row reductions run in parallel over rows and column sums over blocks of
+ * columns, each summing over the rows in order, so the results do not depend
+ * on the thread count. */
+
+/* rows of v_ij = -(W_ij / 2 W_inf + th_j) / etat: max and log sum exp(v - max) */
+static void cl_rows(const double *W, int64_t n, const double *th, double winf, double etat,
+                    double *mx, double *ls) {
+  OMP_FOR
+  for (int64_t i = 0; i < n; ++i) {
+    const double *w = W + i * n;
+    double m = -INFINITY;
+    for (int64_t j = 0; j < n; ++j) {
+      const double v = -(w[j] / (2.0 * winf) + th[j]) / etat;
+      if (v > m) m = v;
+    }
+    double s = 0.0;
+    for (int64_t j = 0; j < n; ++j) s += exp(-(w[j] / (2.0 * winf) + th[j]) / etat - m);
+    mx[i] = m;
+    ls[i] = log(s);
+  }
+}
+
+/* infeas(theta, eta_t) (521-526): column sums of r_i softmax_j(v_ij) */
+static void cl_infeas(const double *W, int64_t n, const double *r, const double *th,
+                      double winf, double etat, double *mx, double *ls, double *res) {
+  cl_rows(W, n, th, winf, etat, mx, ls);
+  const int64_t BS = 64;
+  const int64_t nb = (n + BS - 1) / BS;
+  OMP_FOR
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t j0 = b * BS, j1 = j0 + BS < n ? j0 + BS : n;
+    for (int64_t j = j0; j < j1; ++j) res[j] = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double *w = W + i * n;
+      const double off = -mx[i] - ls[i] + log(r[i]);
+      for (int64_t j = j0; j < j1; ++j) res[j] += exp(-(w[j] / (2.0 * winf) + th[j]) / etat + off);
+    }
+  }
+}
+
+/* theta <- tanh((res - c) ./ eta_mu + (1 - tau_mu eta_mu) / 2 log((th0 + 1) / (1 - th0))) (541, 545) */
+static void cl_theta(double *out, const double *th0, const double *res, const double *c,
+                     const double *etamu, int64_t n, double tau_mu, double eta_mu) {
+  for (int64_t j = 0; j < n; ++j)
+    out[j] = tanh((res[j] - c[j]) / etamu[j] +
+                  0.5 * (1.0 - tau_mu * eta_mu) * log((th0[j] + 1.0) / (1.0 - th0[j])));
+}
+
+int or_cpu_lamp_solve(const double *r, const double *c, int64_t n, const double *W,
+                      const or_args *args, int64_t frequency, or_log_fn logfn, void *user,
+                      double *plan_out, double *theta_out, or_result *out) {
+  if (n <= 0 || frequency <= 0) return 1;
+  double winf = 0.0; /* norm(W, Inf): the largest |W_ij| (508) */
+  for (int64_t k = 0; k < n * n; ++k) winf = fmax(winf, fabs(W[k]));
+  const double etap = args->eta_p / (2.0 * winf); /* 509 */
+  double *theta = calloc((size_t)n, sizeof(double)), *nu = calloc((size_t)n, sizeof(double));
+  double *thetab = calloc((size_t)n, sizeof(double)), *nub = calloc((size_t)n, sizeof(double));
+  double *gam = calloc((size_t)n, sizeof(double)), *res = calloc((size_t)n, sizeof(double));
+  double *mx = calloc((size_t)n, sizeof(double)), *ls = calloc((size_t)n, sizeof(double));
+  double *etamu = calloc((size_t)n, sizeof(double)), *lse = calloc((size_t)n, sizeof(double));
+  double *P = malloc(sizeof(double) * (size_t)n * (size_t)n);
+  double *rs = calloc((size_t)n, sizeof(double)), *cs = calloc((size_t)n, sizeof(double));
+  for (int64_t j = 0; j < n; ++j) etamu[j] = (c[j] + args->alpha / (double)n) / args->tau_mu; /* 523 */
+  double etat = INFINITY;
+  const double taup = args->tau_p;
+  const double minv = tanh(-args->B / 2.0), maxv = tanh(args->B / 2.0);
+  double rlogr = 0.0;
+  for (int64_t i = 0; i < n; ++i) rlogr += r[i] * log(r[i]);
+  emit(logfn, user, "time(s),iter,infeas,ot_objective,primal,dual,solver\n"); /* 527 */
+  const double t0 = now_s();
+  double feas = NAN, obj = NAN, pobj = NAN, dobj = NAN;
+  int converged = 0;
+  int64_t it_exit = 0;
+  for (int64_t it = 1; it <= args->itermax; ++it) {
+    it_exit = it;
+    const double elapsed = now_s() - t0;
+    if (elapsed > args->tmax) break;
+    if ((it - 1) % frequency == 0) { /* 543-557 */
+      /* p = softmax(-(W / 2 W_inf .+ nu') ./ etat, norm_dims = 2) */
+      cl_rows(W, n, nu, winf, etat, mx, ls);
+      OMP_FOR
+      for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < n; ++j)
+          P[i * n + j] = exp(-(W[i * n + j] / (2.0 * winf) + nu[j]) / etat - mx[i] - ls[i]);
+      /* feas = norm(sum(r .* p, dims=1)' - c, 1) */
+      feas = 0.0;
+      for (int64_t j = 0; j < n; ++j) cs[j] = 0.0;
+      for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < n; ++j) cs[j] += r[i] * P[i * n + j];
+      for (int64_t j = 0; j < n; ++j) feas += fabs(cs[j] - c[j]);
+      /* primalv(p, W, W_inf, etap, r, c) (MirrorProx.jl:28-32) */
+      double wp = 0.0, ent = 0.0;
+      for (int64_t i = 0; i < n; ++i) {
+        double e = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+          const double p = P[i * n + j];
+          wp += W[i * n + j] * r[i] * p;
+          if (p > 0.0) e += p * log(p);
+        }
+        ent += r[i] * e;
+      }
+      pobj = 2.0 * (0.5 * wp + winf * feas + etap * ent + etap * rlogr);
+      /* dualv(gamma, W, W_inf, etap, r, c) (MirrorProx.jl:7-13) */
+      if (etap == 0.0) {
+        double a = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+          double m = INFINITY;
+          for (int64_t j = 0; j < n; ++j) m = fmin(m, W[i * n + j] + 2.0 * winf * gam[j]);
+          a += r[i] * m;
+        }
+        dobj = -dotv(c, gam, n) * 2.0 * winf + a;
+      } else {
+        OMP_FOR
+        for (int64_t i = 0; i < n; ++i) {
+          double m = -INFINITY;
+          for (int64_t j = 0; j < n; ++j) m = fmax(m, -(0.5 * W[i * n + j] + winf * gam[j]) / etap);
+          double s = 0.0;
+          for (int64_t j = 0; j < n; ++j) s += exp(-(0.5 * W[i * n + j] + winf * gam[j]) / etap - m);
+          lse[i] = log(s) + m;
+        }
+        dobj = 2.0 * ((-winf * dotv(c, gam, n) - etap * dotv(r, lse, n)) + etap * rlogr);
+      }
+      /* obj = dot(round(r .* p, r, c), W) (Utilities.jl:158-165) */
+      for (int64_t i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int64_t j = 0; j < n; ++j) s += r[i] * P[i * n + j];
+        rs[i] = fmin(r[i] / s, 1.0);  /* row scaling of gamma+ */
+      }
+      for (int64_t j = 0; j < n; ++j) cs[j] = 0.0;
+      for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < n; ++j) cs[j] += r[i] * P[i * n + j] * rs[i];
+      for (int64_t j = 0; j < n; ++j) cs[j] = fmin(c[j] / cs[j], 1.0);  /* column scaling */
+      double l1 = 0.0;
+      double *rmu = lse, *rnu = mx;  /* (reused as scratch) */
+      for (int64_t j = 0; j < n; ++j) rnu[j] = c[j];
+      for (int64_t i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+          const double g = r[i] * P[i * n + j] * rs[i] * cs[j];
+          s += g;
+          rnu[j] -= g;
+        }
+        rmu[i] = r[i] - s;
+        l1 += fabs(rmu[i]);
+      }
+      obj = 0.0;
+      for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < n; ++j)
+          obj += W[i * n + j] * (r[i] * P[i * n + j] * rs[i] * cs[j] + rmu[i] * rnu[j] / l1);
+      if (args->verbose) {
+        char b0[64], b1[64], b2[64], b3[64], b4[64], line[512];
+        fmt_e(b0, 64, "%.6e", elapsed);
+        fmt_e(b1, 64, "%.14e", feas);
+        fmt_e(b2, 64, "%.14e", obj);
+        fmt_e(b3, 64, "%.14e", pobj);
+        fmt_e(b4, 64, "%.14e", dobj);
+        snprintf(line, sizeof line, "%s,%lld,%s,%s,%s,%s,lamp\n", b0, (long long)it, b1, b2, b3, b4);
+        emit(logfn, user, line);
+      }
+      if (pobj - dobj < args->epsilon / 6.0 && feas < args->epsilon / 6.0) { /* 554 */
+        converged = 1;
+        break;
+      }
+    }
+    cl_infeas(W, n, r, nu, winf, etat, mx, ls, res);                         /* 560 */
+    cl_theta(thetab, theta, res, c, etamu, n, args->tau_mu, args->eta_mu);  /* 561 */
+    etat = 1.0 / (taup + (1.0 / etat) * (1.0 - etap));                      /* 562 */
+    for (int64_t j = 0; j < n; ++j) nub[j] = nu[j] + taup * etat * (theta[j] - nu[j]); /* 563 */
+    cl_infeas(W, n, r, nub, winf, etat, mx, ls, res);                       /* 564 */
+    cl_theta(theta, theta, res, c, etamu, n, args->tau_mu, args->eta_mu);   /* 565 */
+    for (int64_t j = 0; j < n; ++j) gam[j] = theta[j] + taup * etat * (theta[j] - gam[j]); /* 566 */
+    for (int64_t j = 0; j < n; ++j) theta[j] = fmin(fmax(theta[j], minv), maxv);            /* 567 */
+    for (int64_t j = 0; j < n; ++j) nu[j] = nu[j] + taup * etat * (thetab[j] - nu[j]);       /* 568 */
+  }
+  /* 570-571: the plan r .* p of the final nu, eta_t */
+  if (plan_out) {
+    cl_rows(W, n, nu, winf, etat, mx, ls);
+    OMP_FOR
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t j = 0; j < n; ++j)
+        plan_out[i * n + j] = r[i] * exp(-(W[i * n + j] / (2.0 * winf) + nu[j]) / etat - mx[i] - ls[i]);
+  }
+  if (theta_out) memcpy(theta_out, theta, sizeof(double) * (size_t)n);
+  if (out) {
+    out->iterations = it_exit;
+    out->objective = obj;
+    out->residual = feas;
+    out->primal = pobj;
+    out->dual = dobj;
+    out->winf = winf;
+    out->eta_t = etat;
+    out->converged = converged;
+  }
+  free(theta); free(nu); free(thetab); free(nub); free(gam); free(res); free(mx); free(ls);
+  free(etamu); free(lse); free(P); free(rs); free(cs);
   return 0;
 }

@@ -167,6 +167,14 @@ int or_lamp_cellsim2_solve(const double *data1, int64_t n, const double *data2,
                            int64_t frequency, or_log_fn logfn, void *user,
                            double *nu_out, double *phi_out, double *psi_out,
                            or_result *out);
+
+/* LAMP(r, c, W::AbstractMatrix, args, frequency) (src/solvers/MirrorProx.jl:
+ * 506-575), the reference's CPU path: dense W (n x n row-major), FP64.
+ * plan (n x n, may be NULL) = r .* p of the final state; theta (n). */
+int or_cpu_lamp_solve(const double *r, const double *c, int64_t n, const double *W,
+                      const or_args *args, int64_t frequency, or_log_fn logfn, void *user,
+                      double *plan, double *theta, or_result *res);
+
 /* sinkhorn_cellsim (src/solvers/SinkhornCellSim.jl:240-310): within each
  * half-step the cost is evaluated among the rows of that call's first data
  * argument (SinkhornCellSim.jl:43, 193); the reference calls it with

@@ -273,6 +273,8 @@ _SIG_F = {
     "or_feature_cost_matrix": (None, [_I, _P, _I64, _P, _I64, _I64, _D, _P]),
     "or_lamp_cellsim_solve": (_I, [_P, _I64, _I64, _D, _I, _P, _P, C.POINTER(or_args), _I64, LOG_FN,
                                    _P, _P, _P, _P, C.POINTER(or_result)]),
+    "or_cpu_lamp_solve": (_I, [_P, _P, _I64, _P, C.POINTER(or_args), _I64, LOG_FN, _P, _P, _P,
+                               C.POINTER(or_result)]),
     "or_lamp_cellsim2_solve": (_I, [_P, _I64, _P, _I64, _I64, _D, _I, _P, _P, C.POINTER(or_args),
                                     _I64, LOG_FN, _P, _P, _P, _P, C.POINTER(or_result)]),
     "or_sinkhorn_cellsim_solve": (_I, [_P, _I64, _P, _I64, _I64, _D, _I, _P, _P, C.POINTER(or_args),
@@ -332,6 +334,22 @@ def lamp_cellsim_solve(data, r, c, metric="cosine", scale=None, frequency=100, *
                                        _p(psi), C.byref(res))
     assert rc == 0
     return dict(nu=nu, phi=phi, psi=psi, log=lg.lines, **res.as_dict())
+
+
+def cpu_lamp_solve(r, c, W, frequency=50, **args):
+    """LAMP(r, c, W::AbstractMatrix, args, frequency) (src/solvers/MirrorProx.jl:
+    506-575), the reference's CPU path, on a dense n x n W: returns the plan
+    r .* p, theta, the log lines and the result fields (iterations is the
+    loop index at exit)."""
+    r, c, W = _f(r), _f(c), np.ascontiguousarray(W, dtype=np.float64)
+    n = r.size
+    assert W.shape == (n, n) and c.size == n
+    plan, theta = np.empty((n, n)), np.empty(n)
+    a, res, lg = make_args(**args), or_result(), _Log()
+    rc = _flib().or_cpu_lamp_solve(_p(r), _p(c), n, _p(W), C.byref(a), int(frequency), lg.cb, None,
+                                   _p(plan), _p(theta), C.byref(res))
+    assert rc == 0
+    return dict(plan=plan, theta=theta, log=lg.lines, **res.as_dict())
 
 
 def lamp_cellsim2_solve(data1, data2, r, c, metric="cosine", scale=None, frequency=100, **args):

@@ -406,3 +406,42 @@ def test_annealed_sinkhorn_ends_at_the_eps_fixed_point(oracle, kind):
     l2 = np.abs(pa - plan(2 * eps, d2["phi"], d2["psi"])).sum()
     assert l1 <= 4 * an["residual"], (l1, an["residual"])
     assert l1 < 0.1 * l2, (l1, l2)
+
+
+def test_cpu_lamp_restatement_lp_limit_and_entropic_fixed_point(oracle):
+    """The reference's CPU LAMP (src/solvers/MirrorProx.jl:506-575), restated
+    in the oracle.  eta_p = 0: its own stop (gap and infeasibility below
+    epsilon/6, tested at every check) ends the solve with the LP cost (HiGHS)
+    and every logged primal / dual bracket the optimum.  eta_p > 0: eta_t
+    tends to eta_p / (2 W_inf) and the plan to the dense Sinkhorn plan at
+    regularisation eta_p (the exponent is -(W / 2 W_inf + nu) / eta_t)."""
+    from scipy.optimize import linprog
+    rng = np.random.default_rng(3)
+    n = 16
+    x = rng.uniform(size=(n, 2))
+    W = ((x[:, None] - x[None]) ** 2).sum(-1)
+    r, c = rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(n))
+    A = np.zeros((2 * n, n * n))
+    for i in range(n):
+        A[i, i * n:(i + 1) * n] = 1
+        A[n + i, i::n] = 1
+    lp = linprog(W.ravel(), A_eq=A, b_eq=np.concatenate([r, c]), bounds=(0, None),
+                 method="highs").fun
+    o = oracle.cpu_lamp_solve(r, c, W, frequency=10, eta_p=0.0, epsilon=6e-6, B=1.1,
+                              itermax=300000, verbose=True)
+    assert o["converged"] == 1
+    assert abs(o["objective"] - lp) < 1e-5 * lp
+    rows = [l.split(",") for l in o["log"][1:]]
+    assert all(q[-1].strip() == "lamp" and len(q) == 7 for q in rows)
+    assert o["log"][0] == "time(s),iter,infeas,ot_objective,primal,dual,solver\n"
+    primal = np.array([float(q[4]) for q in rows])
+    dual = np.array([float(q[5]) for q in rows])
+    assert np.all(dual <= lp + 1e-12) and np.all(primal >= lp - 1e-12)
+    eta = 0.05
+    o = oracle.cpu_lamp_solve(r, c, W, frequency=10, eta_p=eta, epsilon=6e-6, B=1.1,
+                              itermax=3000, verbose=False)
+    assert abs(o["eta_t"] - eta / (2 * W.max())) < 1e-12 * eta
+    d = oracle.sinkhorn_log_dense(r, c, W, frequency=10, eta_p=eta, epsilon=6e-14, itermax=200000,
+                                  verbose=False)
+    assert np.abs(o["plan"] - d["P"]).max() < 1e-10 * d["P"].max()
+    assert abs(o["plan"].sum(0) - c).sum() < 1e-12
