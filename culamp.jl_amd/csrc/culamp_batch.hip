@@ -255,7 +255,7 @@ struct Engine {
       bi.split_len = len;
       bi.splits = (int32_t)splits;
       bi.pbase = pbase;
-      if (splits > 1) pbase += 2 * splits * bi.ntp;
+      if (splits > 1) pbase += (int64_t)pass_fields(KIND_SUM) * splits * bi.ntp;  // (the most fields of LSE / SUM)
       P.insts[k] = bi;
       item_work[k] = len;
     }

@@ -194,7 +194,7 @@ constexpr int kMergeLanes = 4;      // split merge: adjacent lanes sharing a tar
 constexpr int kWideWaves = 16;
 constexpr int kWideThreads = 64 * kWideWaves;
 constexpr int kWideTargets = 64 / kLanesPerTarget;
-constexpr int kMaxFields = 5;   // partial fields per target (BARY)
+constexpr int kMaxFields = 6;   // partial fields per target (BARY)
 
 int pass_fields(int kind);
 LaunchCfg choose_launch(int64_t nt, int64_t ns, int precision, int kind,
@@ -238,26 +238,55 @@ hipError_t launch_batch_mfma(const BatchLaunch& b, int kind, const Work& w, hipS
 // Threads of a reduction workgroup.
 constexpr int kReduceThreads = 1024;
 
-// One element's contribution to reduction `op` (see ReduceSpec).
-__device__ __forceinline__ double reduce_fold(const int op, const double acc, const double x,
-                                              const double y) {
-  if (op == 0) return acc + x;
-  if (op == 1) return acc + fabs(x - y);
-  if (op == 2) return acc + x * y;
-  if (op == 3) return fmax(acc, x);
-  if (op == 4) return x > 0.0 ? acc + x * log(x) : acc;
-  return acc + fabs(x);
+// Compensated FP64 summation (Knuth's TwoSum): s + x is rounded into s and
+// its exact rounding error is added to the running compensation c, so that
+// s + c carries the sum with an error independent of the number of terms
+// (to first order).  Contraction is off: every operation must round as
+// written.
+__device__ __forceinline__ void two_sum_acc(double& s, double& c, const double x) {
+#pragma clang fp contract(off)
+  const double t = s + x;
+  const double bp = t - s;
+  c += (s - (t - bp)) + (x - bp);
+  s = t;
+}
+
+// One element's contribution to reduction `op` (see ReduceSpec), folded into
+// the compensated pair (s, c).  Products are split exactly with an FMA
+// (x * y = p + e), so a dot product is as accurate as in twice the working
+// precision.
+__device__ __forceinline__ void reduce_fold(const int op, double& s, double& c, const double x,
+                                            const double y) {
+#pragma clang fp contract(off)
+  if (op == 3) {
+    s = fmax(s, x);
+  } else if (op == 2) {
+    const double p = x * y;
+    c += fma(x, y, -p);
+    two_sum_acc(s, c, p);
+  } else if (op == 0) {
+    two_sum_acc(s, c, x);
+  } else if (op == 1) {
+    two_sum_acc(s, c, fabs(x - y));
+  } else if (op == 4) {
+    if (x > 0.0) two_sum_acc(s, c, x * log(x));
+  } else {
+    two_sum_acc(s, c, fabs(x));
+  }
 }
 
 // Reduction k of `spec` by one workgroup (the reductions of a spec run as
 // separate workgroups, so their memory round trips overlap): each thread folds
-// four strided elements per round with their loads in flight together, the
-// partials meet by butterfly within each wave and across the waves through
-// LDS; a fixed order, so the result depends only on the input.
+// four strided elements per round with their loads in flight together into a
+// compensated pair (sum, error), the pairs meet by butterfly within each wave
+// and across the waves through LDS, each meeting adding the two sums with
+// TwoSum; the result is sum + error.  A fixed order, so the result depends
+// only on the input.
 __device__ __forceinline__ void reduce_op(const ReduceSpec& spec, const int k, double* out) {
+#pragma clang fp contract(off)
   constexpr int W = kReduceThreads / 64;
   constexpr int U = 4;
-  __shared__ double sh[W];
+  __shared__ double sh[2][W];
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const int op = spec.op[k];
   const double* a = spec.a[k];
@@ -265,7 +294,7 @@ __device__ __forceinline__ void reduce_op(const ReduceSpec& spec, const int k, d
   const int64_t n = spec.len[k];
   const bool two = op == 1 || op == 2;
   const bool mx = op == 3;
-  double v = mx ? -INFINITY : 0.0;
+  double v = mx ? -INFINITY : 0.0, ve = 0.0;
   for (int64_t i0 = tid; i0 < n; i0 += U * kReduceThreads) {
     double xa[U], xb[U];
 #pragma unroll
@@ -280,23 +309,39 @@ __device__ __forceinline__ void reduce_op(const ReduceSpec& spec, const int k, d
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (i0 + (int64_t)u * kReduceThreads < n) v = reduce_fold(op, v, xa[u], xb[u]);
+      if (i0 + (int64_t)u * kReduceThreads < n) reduce_fold(op, v, ve, xa[u], xb[u]);
   }
+  auto meet = [&](const double o, const double oe) {
+    if (mx) {
+      v = fmax(v, o);
+    } else {
+      ve += oe;
+      two_sum_acc(v, ve, o);
+    }
+  };
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     const double o = __shfl_xor(v, off);
-    v = mx ? fmax(v, o) : v + o;
+    const double oe = __shfl_xor(ve, off);
+    meet(o, oe);
   }
-  if (lane == 0) sh[wave] = v;
+  if (lane == 0) {
+    sh[0][wave] = v;
+    sh[1][wave] = ve;
+  }
   __syncthreads();
   if (wave != 0) return;
-  v = lane < W ? sh[lane] : (mx ? -INFINITY : 0.0);
+  v = lane < W ? sh[0][lane] : (mx ? -INFINITY : 0.0);
+  ve = lane < W ? sh[1][lane] : 0.0;
 #pragma unroll
   for (int off = W / 2; off > 0; off >>= 1) {
     const double o = __shfl_xor(v, off);
-    v = mx ? fmax(v, o) : v + o;
+    const double oe = __shfl_xor(ve, off);
+    meet(o, oe);
   }
-  if (lane == 0) out[k] = v;
+  // (v itself is the plain sum in the same order: an infinite or NaN term
+  // leaves it as plain summation would, while its error term is NaN)
+  if (lane == 0) out[k] = (mx || !isfinite(v)) ? v : v + ve;
 }
 
 // out = sa * a + sb * b with both products rounded (no fused multiply-add),

@@ -29,10 +29,11 @@ using namespace culamp_host;
 
 namespace {
 
-// 8 MiB: an automatic launch keeps splits * target blocks below twice its
-// target of 8 workgroups per CU, so its partials (2 fields x splits x padded
-// targets) stay below 2^20 doubles on a 256-CU device.
-constexpr size_t kScratchDoubles = size_t(1) << 20;
+// 16 MiB: an automatic launch keeps splits * target blocks below twice its
+// target of 8 workgroups per CU, so its partials (3 fields -- m, S and the
+// compensation of S -- x splits x 128 padded targets per block) stay below
+// 3 x 4096 x 128 < 2^21 doubles on a 256-CU device.
+constexpr size_t kScratchDoubles = size_t(1) << 21;
 constexpr int kMaxDevices = 64;
 
 static __device__ __align__(256) double scratch[kScratchDoubles];

@@ -79,20 +79,66 @@ __device__ __forceinline__ double pair_cost(double ax, double ay, double az,
   }
 }
 
+// FP64 fields of a partial state (split partials in HBM, the workgroup-wide
+// layout's LDS exchange): MAX / MIN [acc]; SUM [S, Sc, Se, Sce]; LSE
+// [m, S, Se]; BARY [m, S, Se, Sy0, Sy1, Sy2].  The compensation terms travel
+// with their sums, so a merge loses nothing to a partial that is large
+// beside the final sum (cancelling signed terms).
 template <int KIND>
 constexpr int kind_fields() {
-  return KIND == KIND_BARY ? 5 : (KIND == KIND_MAX || KIND == KIND_MIN) ? 1 : 2;
+  return KIND == KIND_BARY ? 6 : (KIND == KIND_MAX || KIND == KIND_MIN) ? 1 : KIND == KIND_SUM ? 4 : 3;
 }
 
 // Per-target accumulator state of a pass.  R is the pair arithmetic type; in
 // mixed precision `m` is an integer-valued FP32 running max in log2 units.
+// Compensated FP64 sums: the marginal and cost sums of SUM passes (both
+// precisions) and the exp-sums of FP64 passes add with TwoSum, the rounding
+// errors going to Se / Sce (scaled with S when the reference moves); merges
+// of partial states are TwoSums too, and the epilogue folds the errors in
+// (settle).  A mixed-precision LSE adds its FP32 sub-tile sums to S plainly:
+// each such sum carries a relative error near 2^-24, against 2^-53 for the
+// FP64 addition, so a compensation could not change its result.
 template <typename R>
 struct TState {
   R m;
   double S, Sc, Sy0, Sy1, Sy2;
   R acc;
   double qd;  // exact value of the per-target exponent offset used
+  double Se = 0.0, Sce = 0.0;
 };
+
+template <typename R>
+__device__ __forceinline__ void settle(TState<R>& st) {
+  st.S += st.Se;
+  st.Sc += st.Sce;
+  st.Se = 0.0;
+  st.Sce = 0.0;
+}
+
+// A state as its partial fields (see kind_fields) and back.
+template <int KIND, typename R>
+__device__ __forceinline__ void to_fields(const TState<R>& st, double (&f)[6]) {
+  if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
+    f[0] = (double)st.acc;
+  } else if constexpr (KIND == KIND_SUM) {
+    f[0] = st.S; f[1] = st.Sc; f[2] = st.Se; f[3] = st.Sce;
+  } else {
+    f[0] = (double)st.m; f[1] = st.S; f[2] = st.Se;
+    if constexpr (KIND == KIND_BARY) { f[3] = st.Sy0; f[4] = st.Sy1; f[5] = st.Sy2; }
+  }
+}
+
+template <int KIND, typename R>
+__device__ __forceinline__ void from_fields(TState<R>& st, const double (&f)[6]) {
+  if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
+    st.acc = (R)f[0];
+  } else if constexpr (KIND == KIND_SUM) {
+    st.S = f[0]; st.Sc = f[1]; st.Se = f[2]; st.Sce = f[3];
+  } else {
+    st.m = (R)f[0]; st.S = f[1]; st.Se = f[2];
+    if constexpr (KIND == KIND_BARY) { st.Sy0 = f[3]; st.Sy1 = f[4]; st.Sy2 = f[5]; }
+  }
+}
 
 template <int PREC, typename R>
 __device__ __forceinline__ R empty_max() {
@@ -108,8 +154,10 @@ __device__ __forceinline__ void combine_state(TState<R>& a, const TState<R>& b) 
   } else if constexpr (KIND == KIND_MIN) {
     a.acc = a.acc < b.acc ? a.acc : b.acc;
   } else if constexpr (KIND == KIND_SUM) {
-    a.S += b.S;
-    a.Sc += b.Sc;
+    a.Se += b.Se;
+    two_sum_acc(a.S, a.Se, b.S);
+    a.Sce += b.Sce;
+    two_sum_acc(a.Sc, a.Sce, b.Sc);
   } else {
     const R mn = a.m > b.m ? a.m : b.m;
     double fa, fb;
@@ -120,7 +168,13 @@ __device__ __forceinline__ void combine_state(TState<R>& a, const TState<R>& b) 
       fa = exp((double)a.m - (double)mn);
       fb = exp((double)b.m - (double)mn);
     }
-    a.S = a.S * fa + b.S * fb;
+    {
+#pragma clang fp contract(off)
+      const double sa = a.S * fa, sb = b.S * fb;
+      a.Se = a.Se * fa + b.Se * fb;
+      a.S = sa;
+      two_sum_acc(a.S, a.Se, sb);
+    }
     if (KIND == KIND_BARY) {
       a.Sy0 = a.Sy0 * fa + b.Sy0 * fb;
       a.Sy1 = a.Sy1 * fa + b.Sy1 * fb;
@@ -139,6 +193,8 @@ __device__ __forceinline__ void combine_lanes(TState<R>& st) {
     o.m = __shfl_xor(st.m, off);
     o.S = __shfl_xor(st.S, off);
     o.Sc = __shfl_xor(st.Sc, off);
+    o.Se = __shfl_xor(st.Se, off);
+    if (KIND == KIND_SUM) o.Sce = __shfl_xor(st.Sce, off);
     o.acc = __shfl_xor(st.acc, off);
     if (KIND == KIND_BARY) {
       o.Sy0 = __shfl_xor(st.Sy0, off);
@@ -228,7 +284,9 @@ __device__ __forceinline__ void emit_record(const PassDesc& d, const int64_t t, 
 
 template <int PREC, int KIND, typename R>
 __device__ __forceinline__ void epilogue_from(const PassDesc& d, const int64_t t,
-                                              const TState<R>& st, const EpiLoad& E) {
+                                              const TState<R>& st_in, const EpiLoad& E) {
+  TState<R> st = st_in;
+  settle(st);
   if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
     d.out0[t] = (double)st.acc;
   } else if constexpr (KIND == KIND_BARY) {
@@ -311,10 +369,8 @@ __device__ __forceinline__ void finish_pass(const PassDesc& d, const Work& w,
 #pragma unroll
     for (int r = 0; r < NT; ++r) {
       if (tidx[r] >= ntp) continue;
-      double f[5];
-      if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) f[0] = (double)st[r].acc;
-      else if constexpr (KIND == KIND_SUM) { f[0] = st[r].S; f[1] = st[r].Sc; }
-      else { f[0] = (double)st[r].m; f[1] = st[r].S; f[2] = st[r].Sy0; f[3] = st[r].Sy1; f[4] = st[r].Sy2; }
+      double f[6];
+      to_fields<KIND, R>(st[r], f);
 #pragma unroll
       for (int q = 0; q < NF; ++q)
         w.partial[((int64_t)q * splits + split) * ntp + tidx[r]] = f[q];
@@ -402,20 +458,10 @@ __device__ __forceinline__ void merge_body(const PassDesc& d, const Work& w, con
 #pragma unroll 4
     for (int q = sub; q < splits; q += kMergeLanes) {
       TState<R> p = st;
-      if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
-        p.acc = (R)w.partial[(int64_t)q * ntp + t];
-      } else if constexpr (KIND == KIND_SUM) {
-        p.S = w.partial[(int64_t)q * ntp + t];
-        p.Sc = w.partial[((int64_t)splits + q) * ntp + t];
-      } else {
-        p.m = (R)w.partial[(int64_t)q * ntp + t];
-        p.S = w.partial[((int64_t)splits + q) * ntp + t];
-        if (NF == 5) {
-          p.Sy0 = w.partial[((int64_t)2 * splits + q) * ntp + t];
-          p.Sy1 = w.partial[((int64_t)3 * splits + q) * ntp + t];
-          p.Sy2 = w.partial[((int64_t)4 * splits + q) * ntp + t];
-        }
-      }
+      double f[6];
+#pragma unroll
+      for (int fi = 0; fi < NF; ++fi) f[fi] = w.partial[((int64_t)fi * splits + q) * ntp + t];
+      from_fields<KIND, R>(p, f);
       combine_state<PREC, KIND, R>(st, p);
     }
   }
@@ -571,35 +617,50 @@ __device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], cons
       if (mn > (double)st.m) {
         const double f = exp((double)st.m - mn);
         st.S *= f;
+        st.Se *= f;
         if (KIND == KIND_BARY) { st.Sy0 *= f; st.Sy1 *= f; st.Sy2 *= f; }
         st.m = (R)mn;
       }
-      double loc = 0.0, l0 = 0.0, l1 = 0.0, l2 = 0.0;
+      // (every term goes into the compensated sum on its own)
+      double l0 = 0.0, l1 = 0.0, l2 = 0.0;
 #pragma unroll
       for (int u = 0; u < NV; ++u) {
         const double e = exp((double)v[u] - mn) * (double)wv[u];
-        loc += e;
+        two_sum_acc(st.S, st.Se, e);
         if (KIND == KIND_BARY) {
           l0 = fma(e, (double)px[u], l0);
           l1 = fma(e, (double)py[u], l1);
           l2 = fma(e, (double)pz[u], l2);
         }
       }
-      st.S += loc;
       if (KIND == KIND_BARY) { st.Sy0 += l0; st.Sy1 += l1; st.Sy2 += l2; }
     }
   } else if constexpr (KIND == KIND_SUM) {
-    R ls = (R)0, lc = (R)0;
+    if constexpr (PREC == 0) {
+      // every term into the compensated sums; the cost term's product is
+      // split exactly (e C = p + fma(e, C, -p))
+#pragma clang fp contract(off)
 #pragma unroll
-    for (int u = 0; u < NV; ++u) {
-      R e;
-      if constexpr (PREC == 1) e = exp2_fast(v[u]) * wv[u];
-      else e = exp(v[u]) * wv[u];
-      ls += e;
-      if constexpr (NEEDC) lc = fma(e, Cv[u], lc);
+      for (int u = 0; u < NV; ++u) {
+        const double e = exp((double)v[u]) * (double)wv[u];
+        two_sum_acc(st.S, st.Se, e);
+        if constexpr (NEEDC) {
+          const double p = e * (double)Cv[u];
+          st.Sce += fma(e, (double)Cv[u], -p);
+          two_sum_acc(st.Sc, st.Sce, p);
+        }
+      }
+    } else {
+      float ls = 0.0f, lc = 0.0f;
+#pragma unroll
+      for (int u = 0; u < NV; ++u) {
+        const float e = exp2_fast(v[u]) * wv[u];
+        ls += e;
+        if constexpr (NEEDC) lc = fmaf(e, Cv[u], lc);
+      }
+      two_sum_acc(st.S, st.Se, (double)ls);
+      if constexpr (NEEDC) two_sum_acc(st.Sc, st.Sce, (double)lc);
     }
-    st.S += (double)ls;
-    st.Sc += (double)lc;
   }
 }
 
@@ -928,10 +989,8 @@ __global__ __launch_bounds__(kWideThreads) void pair_pass_wide(const PassDesc di
                                                      none, tidx, st);
   if (tidx[0] != none) {
     const int tp = (int)(tidx[0] - t0);
-    double f[5];
-    if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) f[0] = (double)st[0].acc;
-    else if constexpr (KIND == KIND_SUM) { f[0] = st[0].S; f[1] = st[0].Sc; }
-    else { f[0] = (double)st[0].m; f[1] = st[0].S; f[2] = st[0].Sy0; f[3] = st[0].Sy1; f[4] = st[0].Sy2; }
+    double f[6];
+    to_fields<KIND, R>(st[0], f);
 #pragma unroll
     for (int q = 0; q < NF; ++q) s_part[q][wave][tp] = f[q];
   }
@@ -947,20 +1006,10 @@ __global__ __launch_bounds__(kWideThreads) void pair_pass_wide(const PassDesc di
   if (ok) {
     for (int q = sub; q < nsplit; q += kMergeLanes) {
       TState<R> p = acc;
-      if constexpr (KIND == KIND_MAX || KIND == KIND_MIN) {
-        p.acc = (R)s_part[0][q][tl];
-      } else if constexpr (KIND == KIND_SUM) {
-        p.S = s_part[0][q][tl];
-        p.Sc = s_part[1][q][tl];
-      } else {
-        p.m = (R)s_part[0][q][tl];
-        p.S = s_part[1][q][tl];
-        if (NF == 5) {
-          p.Sy0 = s_part[2][q][tl];
-          p.Sy1 = s_part[3][q][tl];
-          p.Sy2 = s_part[4][q][tl];
-        }
-      }
+      double f[6];
+#pragma unroll
+      for (int fi = 0; fi < NF; ++fi) f[fi] = s_part[fi][q][tl];
+      from_fields<KIND, R>(p, f);
       combine_state<PREC, KIND, R>(acc, p);
     }
   }
@@ -1044,11 +1093,14 @@ __device__ __forceinline__ void feature_body(const PassDesc& d, const Work& w, c
   using R = typename std::conditional<PREC == 0, double, float>::type;
   constexpr int TT = kFeatTargets, ST = 64, KC = 16, RED = 17;
   constexpr bool kExpKind = (KIND == KIND_LSE || KIND == KIND_SUM);
+  static_assert(KIND != KIND_BARY, "no barycentric pass for feature costs");
   __shared__ R Xs[KC][TT];
   __shared__ R Ys[KC][ST];
   __shared__ R s_a[ST], s_w[ST], s_n[ST], s_m[ST], s_c[ST], s_s[ST];
   __shared__ int s_valid[ST];
   __shared__ double red0[TT][RED], red1[TT][RED];
+  // (compensation terms of the sums in red1 / red0 for the exp kinds)
+  __shared__ double red2[kExpKind ? TT : 1][RED], red3[KIND == KIND_SUM ? TT : 1][RED];
 
   const int tid = threadIdx.x;
   const int ty = tid >> 4, tx = tid & 15;
@@ -1181,9 +1233,12 @@ __device__ __forceinline__ void feature_body(const PassDesc& d, const Work& w, c
     } else if constexpr (KIND == KIND_SUM) {
       red0[tt][tx] = st[a].S;
       red1[tt][tx] = st[a].Sc;
+      red2[tt][tx] = st[a].Se;
+      red3[tt][tx] = st[a].Sce;
     } else {
       red0[tt][tx] = (double)st[a].m;
       red1[tt][tx] = st[a].S;
+      red2[tt][tx] = st[a].Se;
     }
   }
   __syncthreads();
@@ -1204,21 +1259,31 @@ __device__ __forceinline__ void feature_body(const PassDesc& d, const Work& w, c
       for (int q2 = 1; q2 < 16; ++q2) a = KIND == KIND_MAX ? fmax(a, red0[tid][q2]) : fmin(a, red0[tid][q2]);
       fin[0].acc = (R)a;
     } else if constexpr (KIND == KIND_SUM) {
-      double a = 0.0, c = 0.0;
-      for (int q2 = 0; q2 < 16; ++q2) { a += red0[tid][q2]; c += red1[tid][q2]; }
+      double a = 0.0, c = 0.0, ae = 0.0, ce = 0.0;
+      for (int q2 = 0; q2 < 16; ++q2) {
+        ae += red2[tid][q2];
+        two_sum_acc(a, ae, red0[tid][q2]);
+        ce += red3[tid][q2];
+        two_sum_acc(c, ce, red1[tid][q2]);
+      }
       fin[0].S = a;
       fin[0].Sc = c;
+      fin[0].Se = ae;
+      fin[0].Sce = ce;
     } else {
+#pragma clang fp contract(off)
       double M = red0[tid][0];
       for (int q2 = 1; q2 < 16; ++q2) M = fmax(M, red0[tid][q2]);
-      double a = 0.0;
+      double a = 0.0, ae = 0.0;
       for (int q2 = 0; q2 < 16; ++q2) {
         const double f = PREC == 1 ? ldexp(1.0, (int)fmax(red0[tid][q2] - M, -2000.0))
                                    : exp(red0[tid][q2] - M);
-        a += f * red1[tid][q2];
+        ae += f * red2[tid][q2];
+        two_sum_acc(a, ae, f * red1[tid][q2]);
       }
       fin[0].m = (R)M;
       fin[0].S = a;
+      fin[0].Se = ae;
     }
   }
   finish_pass<PREC, KIND, 1, R>(d, w, splits, split, ntp, tidx, fin);
@@ -1582,19 +1647,22 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
       const double f = exp2_small((double)tq - eoff);
       st.S *= f;
       st.Sc *= f;
+      st.Se *= f;
+      st.Sce *= f;
     }
   }
   if constexpr (KIND == KIND_LSE) st.S *= exp2_small(-dl);
   // merge the two half-waves holding the same target
-  if constexpr (KIND == KIND_LSE) {
-    const float mo = __shfl_xor(st.m, 32);
-    const double So = __shfl_xor(st.S, 32);
-    const float mn = fmaxf(st.m, mo);
-    st.S = ldexp(st.S, (int)fmaxf(st.m - mn, -2000.0f)) + ldexp(So, (int)fmaxf(mo - mn, -2000.0f));
-    st.m = mn;
-  } else {
-    st.S += __shfl_xor(st.S, 32);
-    st.Sc += __shfl_xor(st.Sc, 32);
+  {
+    TState<float> o = st;
+    o.m = __shfl_xor(st.m, 32);
+    o.S = __shfl_xor(st.S, 32);
+    o.Se = __shfl_xor(st.Se, 32);
+    if constexpr (KIND != KIND_LSE) {
+      o.Sc = __shfl_xor(st.Sc, 32);
+      o.Sce = __shfl_xor(st.Sce, 32);
+    }
+    combine_state<1, KIND, float>(st, o);
   }
   int64_t tidx[1] = {lane < 32 ? t : ntp};
   TState<float> fin[1] = {st};

@@ -61,8 +61,10 @@ inline unsigned lin_blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
+// FP64 fields per target of a split partial (the sums travel with their
+// compensation terms): BARY 6, SUM 4, LSE 3, MAX / MIN 1.
 int pass_fields(int kind) {
-  return kind == KIND_BARY ? 5 : (kind == KIND_MAX || kind == KIND_MIN) ? 1 : 2;
+  return kind == KIND_BARY ? 6 : (kind == KIND_MAX || kind == KIND_MIN) ? 1 : kind == KIND_SUM ? 4 : 3;
 }
 
 // Launch shape of the point kernel: kLanesPerTarget lanes share a target, so

@@ -199,8 +199,8 @@ int culamp_lse_rows(culamp_ctx *ctx, const double *x, int64_t n,
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): culamp_lse_rows with the same arguments, kernels, launch
  * shape and result bits, except that it calls no hipMalloc / hipFree and no
- * allocating routine.  Its split partials live in a static 8 MiB device array
- * (2^20 doubles) that each GPU the library runs on holds until the process
+ * allocating routine.  Its split partials live in a static 16 MiB device array
+ * (2^21 doubles) that each GPU the library runs on holds until the process
  * ends; a launch that would need more (only a forced shape can, see
  * culamp_ctx_set_launch) is rejected with CULAMP_ERR_INVALID_ARG.  The array
  * is shared by every call on one GPU and keeps the last call's contents:

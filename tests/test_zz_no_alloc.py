@@ -59,7 +59,7 @@ def test_noalloc_matches_lse_rows(gpu_ctx, layout):
 
 
 def test_noalloc_rejects_launch_beyond_scratch(gpu_ctx):
-    """A forced split count whose partials exceed the 2^20-double array is
+    """A forced split count whose partials exceed the 2^21-double array is
     refused on the host before any launch."""
     import culamp_amd
     from culamp_amd import FP64
