@@ -438,6 +438,19 @@ def bench_clouds(ctx, a, log) -> dict:
         # (LAMP's cost still moves between the two precisions' stop checks)
         om, of = rec["mixed"]["objective"], rec["fp64"]["objective"]
         rec["objective_rel_diff"] = abs(om - of) / abs(of)
+        if solver == "lamp":
+            # both precisions after the FP64 solve's iteration count (epsilon 0
+            # never stops; the check at it = K + 1 reports the state after K)
+            k = rec["fp64"]["iterations"]
+            fixed = base.copy(stop="reference", verbose=True, epsilon=0.0, itermax=k + 1)
+            same = {p: pb.lamp(fixed.copy(precision=p), a.frequency, 2.0, log=None,
+                               assignments=False)[-1] for p in ("mixed", "fp64")}
+            rec["same_iterations"] = {
+                "iterations": k,
+                "objective_rel_diff": abs(same["mixed"]["objective"] - same["fp64"]["objective"])
+                / abs(same["fp64"]["objective"]),
+                "infeasibility_mixed": same["mixed"]["residual"],
+                "infeasibility_fp64": same["fp64"]["residual"]}
         log(f"clouds n={n} {solver}: mixed {rec['mixed']['seconds'] * 1e3:.1f} ms "
             f"({rec['mixed']['iterations']} it, {rec['mixed']['dense_eq_GBps']:.0f} GB/s dense-eq), "
             f"fp64 {rec['fp64']['seconds'] * 1e3:.1f} ms ({rec['fp64']['iterations']} it); "

@@ -175,7 +175,9 @@ def test_batch_bf16_cellsim_and_autotune(gpu_ctx, oracle):
     refs = [oracle.sinkhorn_cellsim_solve(x, y, r, c, "cosine", 2.0, frequency=10, eta_p=5e-2,
                                           epsilon=6e-6, itermax=20000, verbose=False)
             for x, y, r, c in inst]
-    for prec, tol in (("bf16x3", 1e-4), ("bf16", 2e-2)):
+    # bf16: the cost moves ~1.4e-5 on these instances and up to 8e-5 over the
+    # 256 cell-similarity instances of the bench; bf16x3 ~4e-8
+    for prec, tol in (("bf16x3", 1e-6), ("bf16", 2e-4)):
         args = culamp_amd.EOTArgs(eta_p=5e-2, epsilon=6e-6, itermax=20000, verbose=False,
                                   precision=prec)
         out = b.sinkhorn(args, 10)

@@ -245,7 +245,8 @@ def test_bf16_sinkhorn_cellsim_converges(gpu_ctx, oracle):
     r, c = rng.dirichlet(np.ones(700)), rng.dirichlet(np.ones(500))
     o = oracle.sinkhorn_cellsim_solve(X, Y, r, c, "cosine", 2.0, frequency=10, eta_p=5e-2,
                                       epsilon=6e-6, itermax=20000, verbose=False)
-    for prec, tol in (("bf16x3", 1e-4), ("bf16", 2e-2)):
+    # (observed: bf16 3.4e-6 on this instance; bf16x3 below 1e-7 on the batch instances)
+    for prec, tol in (("bf16x3", 1e-6), ("bf16", 2e-4)):
         args = culamp_amd.EOTArgs(eta_p=5e-2, epsilon=6e-6, itermax=20000, verbose=False, precision=prec)
         phi, psi, obj, res, nit = culamp_amd.sinkhorn_cellsim(X, Y, 2.0, "cosine", r, c, args, 10,
                                                               return_cuda=True, log=None, ctx=gpu_ctx)

@@ -66,7 +66,7 @@ def test_c_index_fp64_vs_oracle(gpu_ctx, oracle, solver):
 
 
 def test_c_index_bf16_mfma_batch(gpu_ctx):
-    """Cosine cost through the bf16 MFMA batch: distances within 2e-2
+    """Cosine cost through the bf16 MFMA batch: distances within 1e-2
     relative of the FP64 batch (bf16 similarity error 2^-7 on unit rows) and
     a C-index within 0.05 of the one the FP64 distances give."""
     import culamp_amd
@@ -85,6 +85,6 @@ def test_c_index_bf16_mfma_batch(gpu_ctx):
     sw, smin, smax = _expected(cd, pairs, d64)
     ref = (sw - smin) / (smax - smin)
     say("bf16 c-index distances max rel err", err, "c_index", a["c_index"], "fp64", ref)
-    assert err < 2e-2
+    assert err < 1e-2  # (observed 3.4e-3: near-identical cells give small distances)
     assert 0.02 < ref < 0.9  # clusters overlap: the index is not trivially 0
     assert abs(a["c_index"] - ref) < 0.05
