@@ -17,6 +17,23 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libculamp on the GPU)")
 
 
+# Every GPU test runs under a time limit of its own (pytest-timeout, thread
+# method, so that a test stuck inside a GPU call is still ended and named).
+# The whole GPU suite takes about 150 s; the limit leaves room for a first
+# `import torch` on a fresh machine, which can take a minute or two.
+GPU_TEST_SECONDS = 300
+
+
+def pytest_collection_modifyitems(config, items):
+    try:
+        import pytest_timeout  # noqa: F401
+    except ImportError:
+        return
+    for item in items:
+        if item.get_closest_marker("gpu") is not None and item.get_closest_marker("timeout") is None:
+            item.add_marker(pytest.mark.timeout(GPU_TEST_SECONDS, method="thread"))
+
+
 def unhex(a, shape=None):
     v = np.array([float.fromhex(s) for s in a], dtype=np.float64)
     return v.reshape(shape) if shape is not None else v
