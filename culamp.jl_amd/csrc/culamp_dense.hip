@@ -8,10 +8,9 @@
 // Layout: a dense problem holds W (n x m, row-major) and W^T (m x n,
 // row-major), FP64 or FP32 (mixed precision), as the reference keeps W and
 // Wt (MirrorProx.jl:423-424) so that both passes read rows contiguously.  A
-// pass is one wave per output row: lane l reads columns l, l + 64, ...,
-// kDenseUnroll of them per step (each wave load is one contiguous 256- or
-// 512-byte segment), folds them into its state with the pair kernels'
-// arithmetic, and the 64 lanes combine by butterfly.
+// pass is one wave per output row: the lanes read the row in contiguous
+// 16-byte pieces (see dense_rows), fold them into their states with the pair
+// kernels' arithmetic, and the 64 lanes combine by butterfly.
 #include "culamp_host.h"
 #include "culamp_pair_kernel.h"
 
@@ -25,7 +24,7 @@ struct culamp_dense {
   int precision = 0;   // CULAMP_FP64: double W; CULAMP_MIXED: float W
   void* w = nullptr;   // n x m row-major
   void* wt = nullptr;  // m x n row-major
-  void* terms = nullptr;  // per-column (exponent term, weight) of the current pass
+  void* terms = nullptr;  // per-column (exponent term, weight) of a pass, when formed apart
   double winf = 0.0;      // max |W_ij| (norm(W, Inf) of the reference)
 };
 
@@ -33,12 +32,10 @@ namespace culamp {
 namespace {
 
 constexpr int kDenseWaves = 16;   // rows (one per wave) per workgroup
-constexpr int kDenseUnroll = 8;   // columns per lane per step: 8 loads in flight per lane
 
-// Per-column exponent term and weight of a pass, formed once per pass (the
-// pair kernels' source_terms: FP32 head and folded remainder in mixed
-// precision) so that the row kernel's work per element is one fma, one
-// exponential and one accumulation.
+// Per-column exponent term and weight of a pass (the pair kernels' source
+// terms: FP32 head and folded remainder in mixed precision), formed once per
+// pass when many row workgroups would otherwise each form all of them.
 template <int PREC, bool EXPKIND>
 __global__ void dense_terms(const PassDesc d, void* terms) {
   using R = typename std::conditional<PREC == 0, double, float>::type;
@@ -56,24 +53,39 @@ __global__ void dense_terms(const PassDesc d, void* terms) {
 // out1_r = sum_c srcB[c] exp(v_rc) M_rc); KIND_MIN: out_r = min_c v_rc with
 // gamma = 0 (the min-plus of the eta = 0 dual).  M is rows x cols
 // row-major; the PassDesc carries k, gamma/tgtA (as target_offset reads it),
-// nt = rows, ns = cols, out0 and epi; the column terms come from dense_terms.
+// nt = rows, ns = cols, out0 and epi.
 // A workgroup of kDenseWaves waves owns kDenseWaves consecutive rows (one per
-// wave); the column terms of each chunk of kDenseChunk columns are staged in
-// LDS once for all of them, and each wave's matrix loads for the next chunk
-// are issued before the current chunk is folded.
-template <int PREC, int KIND>
+// wave).  Each lane loads E columns per step, VW per load (16-byte loads when
+// the row length allows it), so a wave's step is one contiguous span of
+// 64 E elements, and the loads for the next chunk are issued before the
+// current chunk is folded.  The launch sizes E to the waves a CU holds: with
+// at most one workgroup per CU a step is 8 KB (FP32, E = 32) or 8 KB (FP64,
+// E = 16) per wave, with more workgroups 2 KB or 4 KB (E = 8), about 64 to
+// 128 KB in flight per CU either way (measured on 4096 x 4096 and 16384 x
+// 16384 FP32 matrices: 17.2 against 20.0 us and 198 against 236 us; 16 KB
+// steps at 4096 x 4096 took 22.6 us).  The column
+// terms of each chunk are staged in LDS once for the workgroup's kDenseWaves
+// rows: computed by the workgroup itself (FUSED: the pass is one launch; used
+// when there are few row workgroups) or read from dense_terms' output.  The
+// fold adds 8 terms at a time, as the pair kernels' sub-tiles do.
+template <int PREC, int KIND, int VW, int E, bool FUSED>
 __global__ __launch_bounds__(64 * kDenseWaves) void dense_rows(const PassDesc d, const void* mat,
                                                                const void* terms) {
   using R = typename std::conditional<PREC == 0, double, float>::type;
-  constexpr int U = kDenseUnroll, CH = 64 * U;
-  __shared__ R s_a[CH], s_w[CH];
-  const R* T = (const R*)terms;
+  typedef R RV __attribute__((ext_vector_type(VW)));
+  constexpr bool kExpKind = KIND == KIND_LSE || KIND == KIND_SUM;
+  constexpr int U = E / VW, CH = 64 * U * VW;
+  constexpr int G = 8;  // terms per fold
+  static_assert((U * VW) % G == 0, "whole folds per step");
+  __shared__ __align__(16) R s_a[CH];
+  __shared__ __align__(16) R s_w[CH];
   const R* M = (const R*)mat;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t row = (int64_t)blockIdx.x * kDenseWaves + wave;
   const bool live = row < d.nt;
   const int64_t rw = live ? row : d.nt - 1;
   const int64_t ncols = d.ns;
+  const int64_t nvec = ncols / VW;  // (the launch picks VW dividing ncols)
   const R k = PREC == 0 ? (R)d.k : (R)(d.k * kLog2e);
   TState<R> st;
   R tq;
@@ -92,43 +104,59 @@ __global__ __launch_bounds__(64 * kDenseWaves) void dense_rows(const PassDesc d,
     st.acc = KIND == KIND_MIN ? (R)INFINITY : (R)0;
   }
   const bool needc = KIND == KIND_SUM && d.out1 != nullptr;
-  const R* Mrow = M + rw * ncols;
-  R mv[U];
-  auto fetch = [&](const int64_t c0) {
+  const RV* Mrow = (const RV*)(M + rw * ncols);
+  RV mv[U];
+  auto fetch = [&](const int64_t v0) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) mv[u] = Mrow[min(c0 + 64 * u + lane, ncols - 1)];
+    for (int u = 0; u < U; ++u) mv[u] = Mrow[min(v0 + 64 * u + lane, nvec - 1)];
   };
   fetch(0);
   for (int64_t c0 = 0; c0 < ncols; c0 += CH) {
     __syncthreads();
     for (int i = threadIdx.x; i < CH; i += 64 * kDenseWaves) {
       const int64_t c = c0 + i;
-      const bool ok = c < ncols;
-      s_a[i] = ok ? T[2 * c] : (R)-INFINITY;
-      s_w[i] = ok ? T[2 * c + 1] : (R)0;
+      R a = (R)-INFINITY, w = (R)0;
+      if (c < ncols) {
+        if constexpr (FUSED) {
+          source_terms_v<PREC, kExpKind, R>(d, d.srcA != nullptr ? d.srcA[c] : 0.0,
+                                            d.srcB != nullptr ? d.srcB[c] : 1.0, &a, &w);
+        } else {
+          a = ((const R*)terms)[2 * c];
+          w = ((const R*)terms)[2 * c + 1];
+        }
+      }
+      s_a[i] = a;
+      s_w[i] = w;
     }
     __syncthreads();
-    R cur[U];
+    RV cur[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) cur[u] = mv[u];
-    if (c0 + CH < ncols) fetch(c0 + CH);
-    R v[U], wv[U], dz[U];
+    if (c0 + CH < ncols) fetch((c0 + CH) / VW);
+    // element e of load u is column c0 + (64 u + lane) VW + e
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      v[u] = fma(k, cur[u], s_a[64 * u + lane] + tq);
-      wv[u] = s_w[64 * u + lane];
-      dz[u] = (R)0;
-    }
-    if constexpr (KIND == KIND_MIN) {
+    for (int g0 = 0; g0 < U * VW; g0 += G) {
+      R v[G], wv[G], cc[G], dz[G];
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (c0 + 64 * u + lane < ncols) st.acc = st.acc < v[u] ? st.acc : v[u];
-    } else if (needc) {
-      accumulate<PREC, KIND, U, R, true>(st, v, wv, cur, dz, dz, dz);
-    } else {
-      // (the running max moves every step, not lazily: a lane's columns of
-      // one row span the row's whole exponent range)
-      accumulate<PREC, KIND, U, R, false>(st, v, wv, dz, dz, dz, dz);
+      for (int q = 0; q < G; ++q) {
+        const int u = (g0 + q) / VW, e = (g0 + q) % VW;
+        const int idx = (64 * u + lane) * VW + e;
+        cc[q] = cur[u][e];
+        v[q] = fma(k, cc[q], s_a[idx] + tq);
+        wv[q] = s_w[idx];
+        dz[q] = (R)0;
+        if constexpr (KIND == KIND_MIN)
+          if (c0 + idx < ncols) st.acc = st.acc < v[q] ? st.acc : v[q];
+      }
+      if constexpr (KIND != KIND_MIN) {
+        if (needc) {
+          accumulate<PREC, KIND, G, R, true>(st, v, wv, cc, dz, dz, dz);
+        } else {
+          // (the running max moves every fold, not lazily: a lane's columns
+          // of one row span the row's whole exponent range)
+          accumulate<PREC, KIND, G, R, false>(st, v, wv, dz, dz, dz, dz);
+        }
+      }
     }
   }
   combine_lanes<PREC, KIND, 64, R>(st);
@@ -199,21 +227,47 @@ int run_dense(culamp_ctx* ctx, const culamp_dense* dm, bool transposed, const Pa
               int kind) {
   const unsigned blocks = (unsigned)((d.nt + kDenseWaves - 1) / kDenseWaves);
   const void* mat = transposed ? dm->wt : dm->w;
-  const unsigned tblocks = (unsigned)((d.ns + 255) / 256);
-  void* T = dm->terms;
   const dim3 g(blocks), b(64 * kDenseWaves);
+  // 16-byte loads when every row starts 16-byte aligned (the row length is a
+  // multiple of the vector width; the allocation itself is 256-byte aligned)
+  // Terms formed by each row workgroup while there are at most about two
+  // workgroups per CU (one launch per pass); beyond that the workgroups
+  // would repeat the term work and its reads many times over, so a separate
+  // launch forms them once.
+  const int64_t ncu = ctx->num_cu > 0 ? ctx->num_cu : 256;
+  const bool fused = (int64_t)blocks <= 2 * ncu;
+  const bool few = (int64_t)blocks <= ncu;
+  void* T = dm->terms;
+  auto go = [&](auto prec_c, auto vw_c) {
+    constexpr int P = decltype(prec_c)::value, V = decltype(vw_c)::value;
+    if (!fused) {
+      const unsigned tblocks = (unsigned)((d.ns + 255) / 256);
+      if (kind == KIND_MIN) hipLaunchKernelGGL((dense_terms<P, false>), dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
+      else hipLaunchKernelGGL((dense_terms<P, true>), dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
+    }
+    auto rows = [&](auto kind_c, auto fused_c) {
+      constexpr int K = decltype(kind_c)::value;
+      constexpr bool F = decltype(fused_c)::value;
+      constexpr int EW = V == 1 ? 8 : (P == 0 ? 16 : 32);  // E at most one workgroup per CU
+      if (few) hipLaunchKernelGGL((dense_rows<P, K, V, EW, F>), g, b, 0, ctx->stream, d, mat, T);
+      else hipLaunchKernelGGL((dense_rows<P, K, V, 8, F>), g, b, 0, ctx->stream, d, mat, T);
+    };
+    auto by_kind = [&](auto fused_c) {
+      if (kind == KIND_LSE) rows(std::integral_constant<int, KIND_LSE>{}, fused_c);
+      else if (kind == KIND_SUM) rows(std::integral_constant<int, KIND_SUM>{}, fused_c);
+      else rows(std::integral_constant<int, KIND_MIN>{}, fused_c);
+    };
+    if (fused) by_kind(std::true_type{});
+    else by_kind(std::false_type{});
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
   if (dm->precision == CULAMP_FP64) {
-    if (kind == KIND_MIN) hipLaunchKernelGGL((dense_terms<0, false>), dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
-    else hipLaunchKernelGGL((dense_terms<0, true>), dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
-    if (kind == KIND_LSE) hipLaunchKernelGGL((dense_rows<0, KIND_LSE>), g, b, 0, ctx->stream, d, mat, T);
-    else if (kind == KIND_SUM) hipLaunchKernelGGL((dense_rows<0, KIND_SUM>), g, b, 0, ctx->stream, d, mat, T);
-    else hipLaunchKernelGGL((dense_rows<0, KIND_MIN>), g, b, 0, ctx->stream, d, mat, T);
+    if (d.ns % 2 == 0) go(I0{}, std::integral_constant<int, 2>{});
+    else go(I0{}, I1{});
   } else {
-    if (kind == KIND_MIN) hipLaunchKernelGGL((dense_terms<1, false>), dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
-    else hipLaunchKernelGGL((dense_terms<1, true>), dim3(tblocks), dim3(256), 0, ctx->stream, d, T);
-    if (kind == KIND_LSE) hipLaunchKernelGGL((dense_rows<1, KIND_LSE>), g, b, 0, ctx->stream, d, mat, T);
-    else if (kind == KIND_SUM) hipLaunchKernelGGL((dense_rows<1, KIND_SUM>), g, b, 0, ctx->stream, d, mat, T);
-    else hipLaunchKernelGGL((dense_rows<1, KIND_MIN>), g, b, 0, ctx->stream, d, mat, T);
+    if (d.ns % 4 == 0) go(I1{}, std::integral_constant<int, 4>{});
+    else go(I1{}, I1{});
   }
   HIP_TRY(hipGetLastError());
   ctx->pairs += (double)d.nt * (double)d.ns;
@@ -277,9 +331,9 @@ int culamp_dense_destroy(culamp_dense* dm) {
   dm->ctx->denses.erase(dm);
   (void)hipSetDevice(dm->ctx->device);
   (void)hipStreamSynchronize(dm->ctx->stream);
+  if (dm->terms) (void)hipFree(dm->terms);
   if (dm->w) (void)hipFree(dm->w);
   if (dm->wt) (void)hipFree(dm->wt);
-  if (dm->terms) (void)hipFree(dm->terms);
   delete dm;
   return CULAMP_OK;
 }
