@@ -50,11 +50,11 @@ def pass_bytes(n: int, m: int, b_c: int, b_x: int = 8) -> float:
 MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
 TRAFFIC_BATCH_FILE = os.path.join(ROOT, "profiles", "traffic_mfma_batch_cellsim256.json")
 # VALU issue slots per pair in the batched bf16 row-LSE loop (gfx950 assembly
-# of the offset-folded fast path: per 16 pairs of a lane 16 v_med3, 16 v_fma,
-# 16 v_exp (half rate: two slots each), 16 v_add and 4 more, plus the loop
-# head's address arithmetic): the pass's own VALU bound is the measured FMA
-# issue rate / this
-BATCH_LOOP_SLOTS_PER_PAIR = (16 + 16 + 2 * 16 + 16 + 4 + 4) / 16
+# of the offset-folded fast path: per 16 pairs of a lane 16 v_fma, 16 v_exp
+# (half rate: two slots each), 16 v_add and 4 more, plus the loop head's
+# address arithmetic): the pass's own VALU bound is the measured FMA issue
+# rate / this
+BATCH_LOOP_SLOTS_PER_PAIR = (16 + 2 * 16 + 16 + 4 + 4) / 16
 
 
 def cell_pass_bytes(n: int, m: int, d: int = 128, b_x: int = 2) -> float:

@@ -938,8 +938,10 @@ static float bf16_to_float(uint16_t h) {
 
 // Unit rows for the MFMA path: z_i = sign(s_i) x_i / |x_i| (cosine) or
 // sign(s_i) (x_i - mean_i) / |x_i - mean_i| (correlation), whose dot is the
-// reference's clamped similarity.  Returns false when some row has no
-// direction (the reference's zero-denominator branches), or d > 128.
+// reference's similarity, each scaled by kUnitRowScale before rounding so
+// that the dot of two rounded rows stays inside [-1, 1] with no clamp (the
+// kernel divides kUnitRowScale^2 back out).  Returns false when some row has
+// no direction (the reference's zero-denominator branches), or d > 128.
 static bool pack_unit_rows(const double* x, int64_t n, int64_t d, int metric, int ks,
                            std::vector<uint16_t>& hi, std::vector<uint16_t>& lo) {
   const int64_t dp = 16 * ks;
@@ -955,7 +957,7 @@ static bool pack_unit_rows(const double* x, int64_t n, int64_t d, int metric, in
       q += v * v;
     }
     if (!(q > 0.0) || !(std::isfinite(s)) || s == 0.0) return false;
-    const double f = (s > 0 ? 1.0 : -1.0) / std::sqrt(q);
+    const double f = (s > 0 ? kUnitRowScale : -kUnitRowScale) / std::sqrt(q);
     for (int64_t k = 0; k < d; ++k) {
       const double z = (x[i + k * n] - mu) * f;
       const uint16_t h = bf16_rne((float)z);

@@ -207,6 +207,13 @@ hipError_t launch_pass_fp64(const PassDesc& d, int kind, const LaunchCfg& c,
                             const Work& w, hipStream_t s);
 // Feature passes (64 targets per workgroup; LaunchCfg.rpt is unused).
 constexpr int kFeatTargets = 64;
+// The bf16 unit rows of the MFMA feature path are scaled by this factor
+// before rounding, so that the dot of two rounded rows (each of norm at most
+// kUnitRowScale (1 + 2^-9) < 1) can never leave [-kUnitRowScale^2,
+// kUnitRowScale^2] and needs no clamp to [-1, 1]; the kernel divides the
+// factor back out in its exponent scale.  (1 - 2^-8)^2 is exact in FP32.
+constexpr double kUnitRowScale = 1.0 - 1.0 / 256.0;
+constexpr double kUnitRowScale2 = kUnitRowScale * kUnitRowScale;
 LaunchCfg choose_feature_launch(int64_t nt, int64_t ns, int num_cu, int force_splits);
 hipError_t launch_feature_fp32(const PassDesc& d, int kind, const LaunchCfg& c,
                                const Work& w, hipStream_t s);

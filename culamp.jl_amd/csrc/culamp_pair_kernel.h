@@ -1379,8 +1379,12 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
   const __bf16* Sl = d.sz_lo;
   // exponent: v = k C + a_s + q_t with C = (1 - sim) / scale
   //           = kc * sim + (a_s + k / scale) + q_t,  kc = -k / scale
-  const float kc = (float)(-d.k * kLog2e / d.fscale);
+  // The rows are scaled by kUnitRowScale, so the products give
+  // kUnitRowScale^2 sim: the exponent factor and the cost divide it out.
+  const float kc = (float)(-d.k * kLog2e / d.fscale / kUnitRowScale2);
   const float inv_scale = (float)(1.0 / d.fscale);
+  const float inv_scale_s = (float)(1.0 / (d.fscale * kUnitRowScale2));
+  const float s2 = (float)kUnitRowScale2;
 
   const int jt = lane & 31;
   const int64_t t = tb * 128 + wave * 32 + jt;
@@ -1415,23 +1419,25 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
   double dl = 0.0;
   // The target's exponent offset rides in the MFMA accumulator: the products
   // start from tvec = tinit = offset / kc (the lane's 16 accumulator entries
-  // all belong to its target), so acc = sim + tinit and the exponent is
-  // fma(kc, med3(acc, tinit - 1, tinit + 1), a_s), the clamp of sim to
-  // [-1, 1] shifted: one addition per pair fewer.  The offset actually
-  // applied is kc * tinit, exact in FP64 (eoff); the LSE's dl and the sums'
-  // final rescale account for its difference from the requested offset.
-  // (kc == 0, an infinite regularisation, keeps the offset out of the
-  // accumulator; so does a tiny |kc|, whose tinit could overflow.)
+  // all belong to its target), so acc = s2 sim + tinit and the exponent is
+  // fma(kc, acc, a_s): one addition and no clamp per pair (s2 sim lies in
+  // [-s2, s2] by the row scaling; the cost, where a clamp of sim to [-1, 1]
+  // is part of the reference's definition, uses med3(acc, tinit - s2,
+  // tinit + s2)).  The offset actually applied is kc * tinit, exact in FP64
+  // (eoff); the LSE's dl and the sums' final rescale account for its
+  // difference from the requested offset.  (kc == 0, an infinite
+  // regularisation, keeps the offset out of the accumulator; so does a tiny
+  // |kc|, whose tinit could overflow.)
   const bool fold = fabsf(kc) >= 0x1p-40f;
-  float tinit = 0.0f, lo = -1.0f, hi = 1.0f;
+  float tinit = 0.0f, lo = -s2, hi = s2;
   double eoff = 0.0;
   f32x16_t tvec;
 #pragma unroll
   for (int r = 0; r < 16; ++r) tvec[r] = 0.0f;
   auto set_offset = [&](const float off) {
     tinit = off / kc;
-    lo = tinit - 1.0f;
-    hi = tinit + 1.0f;
+    lo = tinit - s2;
+    hi = tinit + s2;
     eoff = (double)kc * (double)tinit;
 #pragma unroll
     for (int r = 0; r < 16; ++r) tvec[r] = tinit;
@@ -1592,19 +1598,17 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
         float v[16];
         if (fold) {
 #pragma unroll
-          for (int r = 0; r < 16; ++r) v[r] = fmaf(kc, __builtin_amdgcn_fmed3f(acc[r], lo, hi), sa[r]);
+          for (int r = 0; r < 16; ++r) v[r] = fmaf(kc, acc[r], sa[r]);
           lse_lazy_folded(v);
         } else {
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
-            v[r] = fmaf(kc, sim, sa[r] + tqm);
-          }
+          for (int r = 0; r < 16; ++r) v[r] = fmaf(kc, acc[r], sa[r] + tqm);
           lse_lazy<16>(st, v, sw, tq, tqm, dl);
         }
       } else if (d.out1 != nullptr) {
-        // C = (1 - sim) / scale = (1 + tinit - med3(acc, lo, hi)) / scale
-        const float c1 = (1.0f + tinit) * inv_scale;
+        // C = (1 - sim) / scale with sim = (med3(acc, lo, hi) - tinit) / s2,
+        // i.e. the reference's clamp of sim to [-1, 1]
+        const float c1 = fmaf(tinit, inv_scale_s, inv_scale);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           float v[8], wv[8], Cv[8], dz[8];
@@ -1612,10 +1616,10 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
           for (int u = 0; u < 8; ++u) {
             const int r = h * 8 + u;
             const float simc = __builtin_amdgcn_fmed3f(acc[r], lo, hi);
-            Cv[u] = fold ? fmaf(-simc, inv_scale, c1) : (1.0f - simc) * inv_scale;
+            Cv[u] = fmaf(-simc, inv_scale_s, c1);
             wv[u] = sw[r];
             dz[u] = 0.0f;
-            v[u] = fold ? fmaf(kc, simc, sa[r]) : fmaf(kc, simc, sa[r] + tq);
+            v[u] = fold ? fmaf(kc, acc[r], sa[r]) : fmaf(kc, acc[r], sa[r] + tq);
           }
           accumulate<1, KIND, 8, float>(st, v, wv, Cv, dz, dz, dz);
         }
@@ -1625,14 +1629,13 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
 #pragma unroll
           for (int r = 0; r < 16; ++r) {
             dz[r] = 0.0f;
-            v[r] = fmaf(kc, __builtin_amdgcn_fmed3f(acc[r], lo, hi), sa[r]);
+            v[r] = fmaf(kc, acc[r], sa[r]);
           }
         } else {
 #pragma unroll
           for (int r = 0; r < 16; ++r) {
-            const float sim = __builtin_amdgcn_fmed3f(acc[r], -1.0f, 1.0f);
             dz[r] = 0.0f;
-            v[r] = fmaf(kc, sim, sa[r] + tq);
+            v[r] = fmaf(kc, acc[r], sa[r] + tq);
           }
         }
         accumulate<1, KIND, 16, float, false>(st, v, sw, dz, dz, dz, dz);

@@ -85,6 +85,6 @@ def test_c_index_bf16_mfma_batch(gpu_ctx):
     sw, smin, smax = _expected(cd, pairs, d64)
     ref = (sw - smin) / (smax - smin)
     say("bf16 c-index distances max rel err", err, "c_index", a["c_index"], "fp64", ref)
-    assert err < 1e-2  # (observed 3.4e-3: near-identical cells give small distances)
+    assert err < 1e-2  # (observed 3.4e-3 to 4.2e-3: near-identical cells give small distances)
     assert 0.02 < ref < 0.9  # clusters overlap: the index is not trivially 0
     assert abs(a["c_index"] - ref) < 0.05
