@@ -112,6 +112,7 @@ __global__ __launch_bounds__(64 * kDenseWaves) void dense_rows(const PassDesc d,
   };
   fetch(0);
   for (int64_t c0 = 0; c0 < ncols; c0 += CH) {
+    flush_blocks<PREC, KIND, R>(st);  // the previous chunk's block sums
     __syncthreads();
     for (int i = threadIdx.x; i < CH; i += 64 * kDenseWaves) {
       const int64_t c = c0 + i;
@@ -150,16 +151,16 @@ __global__ __launch_bounds__(64 * kDenseWaves) void dense_rows(const PassDesc d,
       }
       if constexpr (KIND != KIND_MIN) {
         if (needc) {
-          accumulate<PREC, KIND, G, R, true>(st, v, wv, cc, dz, dz, dz);
+          accumulate<PREC, KIND, G, R, true, true>(st, v, wv, cc, dz, dz, dz);
         } else {
           // (the running max moves every fold, not lazily: a lane's columns
           // of one row span the row's whole exponent range)
-          accumulate<PREC, KIND, G, R, false>(st, v, wv, dz, dz, dz, dz);
+          accumulate<PREC, KIND, G, R, false, true>(st, v, wv, dz, dz, dz, dz);
         }
       }
     }
   }
-  combine_lanes<PREC, KIND, 64, R>(st);
+  combine_lanes<PREC, KIND, 64, R>(st);  // (flushes the last chunk's block sums)
   if (lane == 0 && live) epilogue_one<PREC, KIND, R>(d, row, st);
 }
 

@@ -93,9 +93,10 @@ constexpr int kind_fields() {
 // mixed precision `m` is an integer-valued FP32 running max in log2 units.
 // Compensated FP64 sums: the marginal and cost sums of SUM passes (both
 // precisions) and the exp-sums of FP64 passes add with TwoSum, the rounding
-// errors going to Se / Sce (scaled with S when the reference moves); merges
-// of partial states are TwoSums too, and the epilogue folds the errors in
-// (settle).  A mixed-precision LSE adds its FP32 sub-tile sums to S plainly:
+// errors going to Se / Sce (scaled with S when the reference moves); in the
+// mixed SUM passes the FP32 sub-tile sums of a tile first meet in a plain
+// FP64 block sum that is TwoSum-added once per tile.  Merges of partial
+// states are TwoSums too, and the epilogue folds the errors in (settle).  A mixed-precision LSE adds its FP32 sub-tile sums to S plainly:
 // each such sum carries a relative error near 2^-24, against 2^-53 for the
 // FP64 addition, so a compensation could not change its result.
 template <typename R>
@@ -105,14 +106,28 @@ struct TState {
   R acc;
   double qd;  // exact value of the per-target exponent offset used
   double Se = 0.0, Sce = 0.0;
+  // Mixed SUM passes with block flushes: the FP32 sub-tile sums of one tile
+  // add plainly here (a handful of terms) and flush_blocks moves the block
+  // into (S, Se) / (Sc, Sce) with TwoSum once per tile.
+  double Sb = 0.0, Scb = 0.0;
 };
+
+template <int PREC, int KIND, typename R>
+__device__ __forceinline__ void flush_blocks(TState<R>& st) {
+  if constexpr (PREC == 1 && KIND == KIND_SUM) {
+    two_sum_acc(st.S, st.Se, st.Sb);
+    two_sum_acc(st.Sc, st.Sce, st.Scb);
+    st.Sb = 0.0;
+    st.Scb = 0.0;
+  }
+}
 
 template <typename R>
 __device__ __forceinline__ void settle(TState<R>& st) {
-  st.S += st.Se;
-  st.Sc += st.Sce;
-  st.Se = 0.0;
-  st.Sce = 0.0;
+  st.S += st.Se + st.Sb;
+  st.Sc += st.Sce + st.Scb;
+  st.Se = st.Sb = 0.0;
+  st.Sce = st.Scb = 0.0;
 }
 
 // A state as its partial fields (see kind_fields) and back.
@@ -187,6 +202,7 @@ __device__ __forceinline__ void combine_state(TState<R>& a, const TState<R>& b) 
 // Butterfly combine across the G adjacent lanes that share a target.
 template <int PREC, int KIND, int G, typename R>
 __device__ __forceinline__ void combine_lanes(TState<R>& st) {
+  flush_blocks<PREC, KIND, R>(st);
 #pragma unroll
   for (int off = 1; off < G; off <<= 1) {
     TState<R> o = st;
@@ -595,7 +611,9 @@ __device__ __forceinline__ void lse_lazy(TState<float>& st, const float (&v)[NV]
 }
 
 // Fold NV exponents of one target into its running state.
-template <int PREC, int KIND, int NV, typename R, bool NEEDC = true>
+// BLOCK (mixed SUM passes): the sub-tile sums go to the block sums Sb / Scb,
+// which the caller flushes into the compensated sums once per tile.
+template <int PREC, int KIND, int NV, typename R, bool NEEDC = true, bool BLOCK = false>
 __device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], const R (&wv)[NV],
                                            const R (&Cv)[NV], const R (&px)[NV],
                                            const R (&py)[NV], const R (&pz)[NV]) {
@@ -658,8 +676,13 @@ __device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], cons
         ls += e;
         if constexpr (NEEDC) lc = fmaf(e, Cv[u], lc);
       }
-      two_sum_acc(st.S, st.Se, (double)ls);
-      if constexpr (NEEDC) two_sum_acc(st.Sc, st.Sce, (double)lc);
+      if constexpr (BLOCK) {
+        st.Sb += (double)ls;
+        if constexpr (NEEDC) st.Scb += (double)lc;
+      } else {
+        two_sum_acc(st.S, st.Se, (double)ls);
+        if constexpr (NEEDC) two_sum_acc(st.Sc, st.Sce, (double)lc);
+      }
     }
   }
 }
@@ -901,7 +924,7 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
 #pragma unroll
           for (int r = 0; r < RPT; ++r) {
             if constexpr (kLazy) lse_lazy<kSub>(st[r], v[r], wv, tq[r], tqm[r], dl[r]);
-            else accumulate<PREC, KIND, kSub, R, kNeedC>(st[r], v[r], wv, Cv[r], px, py, pz);
+            else accumulate<PREC, KIND, kSub, R, kNeedC, true>(st[r], v[r], wv, Cv[r], px, py, pz);
           }
         }
       };
@@ -921,6 +944,8 @@ __device__ __forceinline__ void pair_accum(const PassDesc& d, SrcPt<R>* s_p, R* 
       } else {
         with_cost(std::false_type{});
       }
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) flush_blocks<PREC, KIND, R>(st[r]);
     }
   }
 #pragma unroll
@@ -1621,7 +1646,7 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
             dz[u] = 0.0f;
             v[u] = fold ? fmaf(kc, acc[r], sa[r]) : fmaf(kc, acc[r], sa[r] + tq);
           }
-          accumulate<1, KIND, 8, float>(st, v, wv, Cv, dz, dz, dz);
+          accumulate<1, KIND, 8, float, true, true>(st, v, wv, Cv, dz, dz, dz);
         }
       } else {  // no cost output: marginal only
         float v[16], dz[16];
@@ -1638,9 +1663,10 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
             v[r] = fmaf(kc, acc[r], sa[r] + tq);
           }
         }
-        accumulate<1, KIND, 16, float, false>(st, v, sw, dz, dz, dz, dz);
+        accumulate<1, KIND, 16, float, false, true>(st, v, sw, dz, dz, dz, dz);
       }
     }
+    flush_blocks<1, KIND, float>(st);
     if (more) store_terms(base + TILE, buf ^ 1);
     __syncthreads();
   }
@@ -1657,6 +1683,7 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
   if constexpr (KIND == KIND_LSE) st.S *= exp2_small(-dl);
   // merge the two half-waves holding the same target
   {
+    flush_blocks<1, KIND, float>(st);
     TState<float> o = st;
     o.m = __shfl_xor(st.m, 32);
     o.S = __shfl_xor(st.S, 32);
