@@ -365,6 +365,66 @@ def bench_dense(ctx, a, log) -> dict:
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_lse_mixed_grid64.json")
 
 
+def copy_ceiling(ctx, log, mib: int = 512, reps: int = 10) -> dict:
+    """Measured HBM stream ceiling on the box: a device-to-device copy of
+    `mib` MiB (read + write) timed with HIP events on the library's stream."""
+    import culamp_amd
+    n = (mib << 20) // 8
+    src, dst = culamp_amd.DeviceArray(ctx, n), culamp_amd.DeviceArray(ctx, n)
+    for _ in range(2):
+        culamp_amd._lib.check(ctx.lib.culamp_memcpy_dtod(ctx.handle, dst.ptr, src.ptr, n * 8))
+    ctx.timer_start()
+    for _ in range(reps):
+        culamp_amd._lib.check(ctx.lib.culamp_memcpy_dtod(ctx.handle, dst.ptr, src.ptr, n * 8))
+    ms = ctx.timer_stop() / reps
+    gbs = 2 * n * 8 / (ms * 1e-3) / 1e9
+    log(f"copy ceiling: {mib} MiB device-to-device in {ms * 1e3:.1f} us, {gbs:.0f} GB/s read + write")
+    return {"bytes_moved": 2 * n * 8, "us": ms * 1e3, "GBps": gbs, "frac_of_8TBps": gbs / HBM_PEAK_GBS}
+
+
+def bench_shard(ctx, a, log) -> dict:
+    """configs[4] at one GPU's share of an 8-GPU row sharding: the row LSE of
+    8192 of the 65,536 source points against all 65,536 targets (the pass each
+    GPU would run per half step; the all-gather of the other side's 65,536
+    FP64 potentials, 512 KiB, is not run), mixed and FP64, graph-replayed and
+    timed with HIP events."""
+    import numpy as np
+    import culamp_amd
+    from culamp_amd import kernels as K
+    from culamp_amd import workloads
+    x, y, _, _ = workloads.divergence_clouds(65536, 65536)
+    rows = 65536 // 8
+    xs = np.ascontiguousarray(x[:rows])
+    C = ((x[:64, None, :] - y[None, :4096, :]) ** 2).sum(-1)
+    reg = 1e-2  # of the cost scale below
+    winf = float(C.max())
+    th = np.zeros(len(y))
+    dx = culamp_amd.DeviceArray.from_host(ctx, xs)
+    dy = culamp_amd.DeviceArray.from_host(ctx, y)
+    dth = culamp_amd.DeviceArray.from_host(ctx, th)
+    out = culamp_amd.DeviceArray(ctx, rows)
+    rec = {"workload": f"row LSE of {rows} of the 65,536 divergence source points (N(0, I3)) against "
+                       "all 65,536 targets (4-component GMM), squared Euclidean, reg = 1e-2 of the "
+                       "sampled max cost: one GPU's pass of an 8-GPU row sharding",
+           "rows": rows, "cols": len(y), "allgather_bytes_per_half_step": len(y) * 8}
+    for prec, name in ((culamp_amd.MIXED, "mixed"), (culamp_amd.FP64, "fp64")):
+        def one():
+            K.warp_logsumexp_spp_ct_fused(ctx, out, dx, dy, dth, reg, 1.0, winf, 2.0, prec)
+        for _ in range(3):
+            one()
+        g = ctx.capture(lambda: [one() for _ in range(5)])
+        g.launch()
+        ctx.timer_start()
+        for _ in range(4):
+            g.launch()
+        ms = ctx.timer_stop() / 20
+        g.close()
+        rec[name] = {"us": ms * 1e3, "pairs_per_s": rows * len(y) / (ms * 1e-3)}
+        log(f"shard {rows}x{len(y)} {name}: {ms * 1e3:.1f} us per row LSE, "
+            f"{rows * len(y) / (ms * 1e-3) / 1e12:.2f} T pairs/s")
+    return rec
+
+
 def bench_divergence(ctx, a, log) -> dict:
     """Differentiable Sinkhorn divergence: x ~ N(0, I_3) (n points), y a
     4-component Gaussian mixture, Uniform(0.5, 1.5) weights; squared
@@ -846,6 +906,8 @@ def main() -> None:
     grid = None if a.no_grid else bench_grid64(ctx, a, log)
     clouds = None if a.no_clouds else bench_clouds(ctx, a, log)
     divergence = None if a.no_diff else bench_divergence(ctx, a, log)
+    shard = None if a.no_diff else bench_shard(ctx, a, log)
+    roofline["copy_ceiling"] = copy_ceiling(ctx, log)
     cidx = None if a.no_cindex else bench_cindex(ctx, a, log)
     dense = None if a.no_dense else bench_dense(ctx, a, log)
 
@@ -869,7 +931,7 @@ def main() -> None:
         "batched": extras,
     }
     for k, v in (("cpu_reference_path", cpu_ref), ("table1", table1), ("grid64", grid),
-                 ("clouds", clouds), ("divergence", divergence),
+                 ("clouds", clouds), ("divergence", divergence), ("shard_one_gpu", shard),
                  ("cindex", cidx), ("dense_comparator", dense)):
         if v is not None:
             rec[k] = v
