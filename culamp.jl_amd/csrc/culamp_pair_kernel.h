@@ -42,6 +42,19 @@ namespace {
 constexpr double kLog2e = 1.4426950408889634074;
 constexpr double kLn2 = 0.69314718055994530942;
 constexpr int kSub = 8;  // sources per sub-tile
+// fourth word of a raw (stride 0) CDNA buffer resource: 32-bit data format,
+// no swizzle; with it the hardware checks each offset against num_records
+constexpr int kBufferWord3 = 0x00020000;
+
+// 16-byte copies from global memory straight into LDS (buffer_load ... lds):
+// `base` and `nbytes` are wave-uniform (the resource is built in scalar
+// registers), `voff` is the lane's byte offset, voff + 16 <= nbytes; lane l
+// writes bytes [16 l, 16 l + 16) from `lds`.
+__device__ __forceinline__ void lds_copy16(const void* base, const int nbytes, const unsigned voff,
+                                           __attribute__((address_space(3))) void* lds) {
+  const auto r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, nbytes, kBufferWord3);
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, lds, 16, voff, 0, 0, 0);
+}
 
 __device__ __forceinline__ float exp2_fast(float x) {
   return __builtin_amdgcn_exp2f(x);
@@ -1380,19 +1393,19 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
   constexpr int DP = 16 * KS;          // padded row width (bf16)
   constexpr int CPR = 2 * KS;          // 16-byte chunks per row
   constexpr int SUB = 32;              // sources per MFMA tile
-  // sources staged per workgroup barrier (two MFMA tiles): 2 x 32 KB of
+  // sources staged per workgroup barrier (two MFMA tiles): 2 x 16 KB of
   // rows per workgroup at d = 128, so that LDS leaves room for three
   // workgroups per CU
   constexpr int TILE = 64;
   // Double-buffered staging: the rows of tile T + 1 go from global memory
-  // straight into the other LDS buffer (global_load_lds, no registers) while
+  // straight into the other LDS buffer (buffer loads to LDS, no registers) while
   // tile T is computed, and the tile's one barrier waits for them.  Rows are
   // unpadded with their 16-byte chunks rotated by an XOR with the row index
   // (chunk c of row r sits at position c ^ (r mod CPR)), so the MFMA operand
   // reads of 16 consecutive rows hit distinct banks.  A partial tile repeats
   // its last row, whose copies carry the exponent term -inf.
-  __shared__ __align__(16) __bf16 Ah[2][TILE * DP];
-  __shared__ __align__(16) __bf16 Al[2][NSPLIT == 3 ? TILE * DP : 8];
+  __shared__ __align__(16) __bf16 Ah[2 * TILE * DP];  // buffer b at b * TILE * DP
+  __shared__ __align__(16) __bf16 Al[NSPLIT == 3 ? 2 * TILE * DP : 8];
   __shared__ __align__(16) float s_a[2][TILE];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -1478,9 +1491,9 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
   // reference m moves as in lse_lazy, tinit follows it and the sub-tile is
   // re-evaluated with the new offset.
   auto lse_lazy_folded = [&](const float (&v)[16]) {
-    float loc = 0.0f;
+    float loc = exp2_fast(v[0]);
 #pragma unroll
-    for (int u = 0; u < 16; ++u) loc += exp2_fast(v[u]);
+    for (int u = 1; u < 16; ++u) loc += exp2_fast(v[u]);
     const double Sn = st.S + (double)loc;
     if (loc < 0x1p100f && Sn > 0x1p-90) {
       st.S = Sn;
@@ -1554,27 +1567,41 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
     const unsigned row = e / CPR, c = (e % CPR) ^ (row % CPR);
     roff[j] = row * (unsigned)(DP * 2) + c * 16u;
   }
+  // The copies are raw buffer loads to LDS: the tile's row base is the
+  // resource (scalar), the lane's 32-bit offset the only vector operand.
+  constexpr int kRowBytes = DP * 2;
   auto load_rows = [&](const int64_t base, const int buf) {
     const int cnt = (int)min((int64_t)TILE, s1 - base);
     const char* th = (const char*)(Sh + base * DP);
     const char* tl = NSPLIT == 3 ? (const char*)(Sl + base * DP) : nullptr;
+    typedef __attribute__((address_space(3))) void* lds_t;
+    if (cnt == TILE) {
+      // full tile: the precomputed lane offsets as they are (no VALU)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (BLOCKS % (kThreads / 64) != 0 && 4 * j + (int)wv_u >= BLOCKS) break;
+        const int blk = (4 * j + (int)wv_u) * 512;  // bf16 elements per 1 KB block
+        lds_copy16(th, TILE * kRowBytes, roff[j], (lds_t)(&Ah[buf * TILE * DP + blk]));
+        if constexpr (NSPLIT == 3)
+          lds_copy16(tl, TILE * kRowBytes, roff[j], (lds_t)(&Al[buf * TILE * DP + blk]));
+      }
+      return;
+    }
+    // a partial tile repeats its last row (every offset stays below cnt rows)
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (BLOCKS % (kThreads / 64) != 0 && 4 * j + (int)wv_u >= BLOCKS) break;
       unsigned off = roff[j];
-      if (cnt < TILE) {  // a partial tile repeats its last row
-        const unsigned row = off / (unsigned)(DP * 2);
-        if (row >= (unsigned)cnt) off += ((unsigned)cnt - 1u - row) * (unsigned)(DP * 2);
-      }
-      const int blk = (4 * j + (int)wv_u) * 512;  // bf16 elements per 1 KB block
-      __builtin_amdgcn_global_load_lds((const void*)(th + off),
-                                       (__attribute__((address_space(3))) void*)(&Ah[buf][blk]),
-                                       16, 0, 0);
+      const unsigned row = off / (unsigned)kRowBytes;
+      if (row >= (unsigned)cnt) off += ((unsigned)cnt - 1u - row) * (unsigned)kRowBytes;
+      const int blk = (4 * j + (int)wv_u) * 512;
+      lds_copy16(th, cnt * kRowBytes, off, (lds_t)(&Ah[buf * TILE * DP + blk]));
       if constexpr (NSPLIT == 3)
-        __builtin_amdgcn_global_load_lds((const void*)(tl + off),
-                                         (__attribute__((address_space(3))) void*)(&Al[buf][blk]),
-                                         16, 0, 0);
+        lds_copy16(tl, cnt * kRowBytes, off, (lds_t)(&Al[buf * TILE * DP + blk]));
     }
+    // (an empty statement the compiler cannot move: it keeps these copies
+    // from being merged with the full tile's into per-lane selects)
+    __asm__ volatile("");
   };
 
   const int ntiles = s1 > s0 ? (int)((s1 - s0 + TILE - 1) / TILE) : 0;
@@ -1584,26 +1611,34 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
     store_terms(s0, 0);
   }
   __syncthreads();
-  for (int T = 0; T < ntiles; ++T) {
-    const int64_t base = s0 + (int64_t)T * TILE;
-    const int buf = T & 1;
-    const bool more = T + 1 < ntiles;
+  // One tile: the buffer index and the sub-tile offsets are compile-time
+  // constants (the tile loop below is unrolled by the two buffers), so the
+  // LDS operand reads carry them as immediate offsets and the loop spends no
+  // VALU on addresses.
+  static_assert(SUB % CPR == 0, "row rotation repeats within a sub-tile");
+  // lane's element offset of its operand chunk ks in row jt of a tile
+  int aoff[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) aoff[ks] = jt * DP + ((2 * ks + (lane >> 5)) ^ (jt & (CPR - 1))) * 8;
+  auto tile = [&](const int64_t base, auto bufc, const bool more) {
+    constexpr int buf = decltype(bufc)::value;
     if (more) {  // in flight during this tile's work
       load_rows(base + TILE, buf ^ 1);
       load_terms(base + TILE);
     }
     const int nsub = ((int)min((int64_t)TILE, s1 - base) + SUB - 1) / SUB;
-    for (int sub = 0; sub < nsub; ++sub) {
+#pragma unroll
+    for (int sub = 0; sub < TILE / SUB; ++sub) {
+      if (sub >= nsub) break;
       const int r0 = sub * SUB;
       f32x16_t acc = tvec;
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
-        const int row = r0 + jt;
-        const int at = row * DP + ((2 * ks + (lane >> 5)) ^ (row & (CPR - 1))) * 8;
-        const bf16x8_t ah = *(const bf16x8_t*)(&Ah[buf][at]);
+        const int at = (buf * TILE + r0) * DP + aoff[ks];
+        const bf16x8_t ah = *(const bf16x8_t*)(&Ah[at]);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[ks], acc, 0, 0, 0);
         if constexpr (NSPLIT == 3) {
-          const bf16x8_t al = *(const bf16x8_t*)(&Al[buf][at]);
+          const bf16x8_t al = *(const bf16x8_t*)(&Al[at]);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[ks], acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[ks], acc, 0, 0, 0);
         }
@@ -1669,6 +1704,11 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
     flush_blocks<1, KIND, float>(st);
     if (more) store_terms(base + TILE, buf ^ 1);
     __syncthreads();
+  };
+  for (int T = 0; T < ntiles; T += 2) {
+    const int64_t base = s0 + (int64_t)T * TILE;
+    tile(base, std::integral_constant<int, 0>{}, T + 1 < ntiles);
+    if (T + 1 < ntiles) tile(base + TILE, std::integral_constant<int, 1>{}, T + 2 < ntiles);
   }
   if constexpr (KIND != KIND_LSE) {
     // back to the requested offset tq (as the split merge and epilogue use)
