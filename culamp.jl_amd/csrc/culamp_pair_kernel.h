@@ -1404,8 +1404,11 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
   // (chunk c of row r sits at position c ^ (r mod CPR)), so the MFMA operand
   // reads of 16 consecutive rows hit distinct banks.  A partial tile repeats
   // its last row, whose copies carry the exponent term -inf.
-  __shared__ __align__(16) __bf16 Ah[2 * TILE * DP];  // buffer b at b * TILE * DP
-  __shared__ __align__(16) __bf16 Al[NSPLIT == 3 ? 2 * TILE * DP : 8];
+  // (the two buffers are separate arrays, so that the compiler sees that
+  // the copies into one never alias the operand reads from the other and
+  // lets them stay in flight)
+  __shared__ __align__(16) __bf16 Ah0[TILE * DP], Ah1[TILE * DP];
+  __shared__ __align__(16) __bf16 Al0[NSPLIT == 3 ? TILE * DP : 8], Al1[NSPLIT == 3 ? TILE * DP : 8];
   __shared__ __align__(16) float s_a[2][TILE];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -1581,9 +1584,9 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
       for (int j = 0; j < NB; ++j) {
         if (BLOCKS % (kThreads / 64) != 0 && 4 * j + (int)wv_u >= BLOCKS) break;
         const int blk = (4 * j + (int)wv_u) * 512;  // bf16 elements per 1 KB block
-        lds_copy16(th, TILE * kRowBytes, roff[j], (lds_t)(&Ah[buf * TILE * DP + blk]));
+        lds_copy16(th, TILE * kRowBytes, roff[j], (lds_t)(&(buf ? Ah1 : Ah0)[blk]));
         if constexpr (NSPLIT == 3)
-          lds_copy16(tl, TILE * kRowBytes, roff[j], (lds_t)(&Al[buf * TILE * DP + blk]));
+          lds_copy16(tl, TILE * kRowBytes, roff[j], (lds_t)(&(buf ? Al1 : Al0)[blk]));
       }
       return;
     }
@@ -1595,9 +1598,9 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
       const unsigned row = off / (unsigned)kRowBytes;
       if (row >= (unsigned)cnt) off += ((unsigned)cnt - 1u - row) * (unsigned)kRowBytes;
       const int blk = (4 * j + (int)wv_u) * 512;
-      lds_copy16(th, cnt * kRowBytes, off, (lds_t)(&Ah[buf * TILE * DP + blk]));
+      lds_copy16(th, cnt * kRowBytes, off, (lds_t)(&(buf ? Ah1 : Ah0)[blk]));
       if constexpr (NSPLIT == 3)
-        lds_copy16(tl, cnt * kRowBytes, off, (lds_t)(&Al[buf * TILE * DP + blk]));
+        lds_copy16(tl, cnt * kRowBytes, off, (lds_t)(&(buf ? Al1 : Al0)[blk]));
     }
     // (an empty statement the compiler cannot move: it keeps these copies
     // from being merged with the full tile's into per-lane selects)
@@ -1622,9 +1625,10 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
   for (int ks = 0; ks < KS; ++ks) aoff[ks] = jt * DP + ((2 * ks + (lane >> 5)) ^ (jt & (CPR - 1))) * 8;
   auto tile = [&](const int64_t base, auto bufc, const bool more) {
     constexpr int buf = decltype(bufc)::value;
-    if (more) {  // in flight during this tile's work
-      load_rows(base + TILE, buf ^ 1);
+    if (more) {  // in flight during this tile's work (the terms first, so
+                 // that their wait at the tile's end leaves the rows in flight)
       load_terms(base + TILE);
+      load_rows(base + TILE, buf ^ 1);
     }
     const int nsub = ((int)min((int64_t)TILE, s1 - base) + SUB - 1) / SUB;
 #pragma unroll
@@ -1634,11 +1638,11 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
       f32x16_t acc = tvec;
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
-        const int at = (buf * TILE + r0) * DP + aoff[ks];
-        const bf16x8_t ah = *(const bf16x8_t*)(&Ah[at]);
+        const int at = r0 * DP + aoff[ks];
+        const bf16x8_t ah = *(const bf16x8_t*)(&(buf ? Ah1 : Ah0)[at]);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[ks], acc, 0, 0, 0);
         if constexpr (NSPLIT == 3) {
-          const bf16x8_t al = *(const bf16x8_t*)(&Al[at]);
+          const bf16x8_t al = *(const bf16x8_t*)(&(buf ? Al1 : Al0)[at]);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[ks], acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[ks], acc, 0, 0, 0);
         }
