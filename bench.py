@@ -382,6 +382,16 @@ def copy_ceiling(ctx, log, mib: int = 512, reps: int = 10) -> dict:
     return {"bytes_moved": 2 * n * 8, "us": ms * 1e3, "GBps": gbs, "frac_of_8TBps": gbs / HBM_PEAK_GBS}
 
 
+def mfma_ceiling(ctx, roofline: dict, log) -> dict:
+    """Measured dense bf16 MFMA ceiling on the box (32x32x16 MFMA chains at
+    three waves per SIMD, the pair pass's occupancy), and the batched pass's
+    fraction of it."""
+    tf = ctx.probe_mfma() / 1e12
+    log(f"mfma ceiling: {tf:.0f} TFLOP/s bf16 sustained ({tf / MFMA_PEAK_TFLOPS:.2f} of the published peak)")
+    return {"TFLOPs": round(tf, 1), "frac_of_peak": round(tf / MFMA_PEAK_TFLOPS, 4),
+            "pass_frac_of_ceiling": round(roofline["achieved"] / tf, 4) if tf > 0 else None}
+
+
 def bench_shard(ctx, a, log) -> dict:
     """configs[4] at one GPU's share of an 8-GPU row sharding: the row LSE of
     8192 of the 65,536 source points against all 65,536 targets (the pass each
@@ -908,6 +918,7 @@ def main() -> None:
     divergence = None if a.no_diff else bench_divergence(ctx, a, log)
     shard = None if a.no_diff else bench_shard(ctx, a, log)
     roofline["copy_ceiling"] = copy_ceiling(ctx, log)
+    roofline["mfma_ceiling"] = mfma_ceiling(ctx, roofline, log)
     cidx = None if a.no_cindex else bench_cindex(ctx, a, log)
     dense = None if a.no_dense else bench_dense(ctx, a, log)
 

@@ -101,6 +101,7 @@ SIGNATURES = {
     "culamp_ctx_set_source_records": (_I, [_P, _I]),
     "culamp_ctx_set_fp64_fallback": (_I, [_P, _I]),
     "culamp_probe_rates": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "culamp_probe_mfma": (_I, [_P, C.POINTER(C.c_double)]),
     "culamp_malloc": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
     "culamp_free": (_I, [_P, _P]),
     "culamp_memcpy_htod": (_I, [_P, _P, _P, C.c_size_t]),
@@ -279,6 +280,13 @@ class Context:
         e, f = C.c_double(), C.c_double()
         check(self.lib.culamp_probe_rates(self.handle, C.byref(e), C.byref(f)))
         return {"exp_per_s": e.value, "fma_per_s": f.value}
+
+    def probe_mfma(self) -> float:
+        """Measured sustained dense bf16 MFMA throughput of the device, flop/s
+        (32x32x16 MFMA chains at three waves per SIMD)."""
+        r = C.c_double()
+        check(self.lib.culamp_probe_mfma(self.handle, C.byref(r)))
+        return r.value
 
     def capture(self, fn) -> "Graph":
         """Records the kernel-level calls made by fn() into a replayable graph."""

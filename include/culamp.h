@@ -162,6 +162,11 @@ int culamp_ctx_set_fp64_fallback(culamp_ctx *ctx, int enable);
  * and of the FP32 fused multiply-add, in lane operations per second, from
  * independent-chain micro-kernels timed on the context's stream (best of 3). */
 int culamp_probe_rates(culamp_ctx *ctx, double *exp_per_s, double *fma_per_s);
+/* MFMA-rate probe (no reference counterpart; measurement): sustained dense
+ * bf16 matrix-core throughput in flop/s, from independent chains of
+ * v_mfma_f32_32x32x16_bf16 at three waves per SIMD (the occupancy of the
+ * MFMA pair pass) on every CU, timed on the context's stream (best of 3). */
+int culamp_probe_mfma(culamp_ctx *ctx, double *bf16_flops_per_s);
 /* Device memory helpers (stream-ordered on the context's stream). */
 int culamp_malloc(culamp_ctx *ctx, size_t bytes, void **dptr);
 int culamp_free(culamp_ctx *ctx, void *dptr);

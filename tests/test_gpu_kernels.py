@@ -415,3 +415,7 @@ def test_probe_rates(gpu_ctx):
     say("probe", r)
     assert 1e11 < r["exp_per_s"] < 1e15 and 1e11 < r["fma_per_s"] < 1e15
     assert r["exp_per_s"] <= 1.05 * r["fma_per_s"]
+    # bf16 MFMA: between a tenth of and the published dense peak (2.5 PFLOP/s)
+    m = gpu_ctx.probe_mfma()
+    say("probe mfma", m / 1e12, "TFLOP/s")
+    assert 2.5e14 < m <= 2.6e15
