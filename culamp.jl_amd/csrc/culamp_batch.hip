@@ -1186,7 +1186,9 @@ int culamp_batch_time_pass(culamp_ctx* ctx, culamp_batch* b, int precision, doub
   Engine E(ctx, b);
   TRY(E.set_active(all_instances(b)));
   TRY(E.prepare(precision, KIND_LSE, true));
-  TRY(E.launch(D.p, precision, KIND_LSE, true, pp));  // warm-up
+  // warm-up: as many launches as are timed (the device's clocks settle over
+  // the first tens of milliseconds of a sustained load)
+  for (int q = 0; q < std::max(reps, 1); ++q) TRY(E.launch(D.p, precision, KIND_LSE, true, pp));
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
   for (int q = 0; q < reps; ++q) TRY(E.launch(D.p, precision, KIND_LSE, true, pp));
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));

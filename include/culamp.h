@@ -411,7 +411,8 @@ int culamp_batch_autotune(culamp_ctx *ctx, culamp_batch *batch, int precision,
                           double *ms_per_pass);
 /* Measurement (no reference counterpart): times the batch's row
  * log-sum-exp pass over all instances at its current tuning, `reps` launches
- * between two HIP events on the context's stream after one warm-up launch;
+ * between two HIP events on the context's stream after `reps` untimed
+ * warm-up launches;
  * *ms_per_pass gets the mean and *launches_per_pass (may be NULL) the kernel
  * launches per pass (2 when some instance's source splits need the merge
  * kernel, else 1). */
