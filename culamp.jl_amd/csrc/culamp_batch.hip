@@ -221,22 +221,22 @@ struct Engine {
   void shape(Plan& P) {
     const int tpb = targets_per_block(P.family, P.rpt);
     const int64_t g = split_granule(P.family);
-    int64_t L;
-    if (b->tune.max_split > 0) {
-      L = b->tune.max_split;
-    } else {
-      L = 0;
-      for (int32_t k : active) L = std::max(L, ns_of(k, P.rows));
-      auto count = [&](int64_t len) {
-        int64_t tot = 0;
-        for (int32_t k : active) {
-          const int64_t nt = nt_of(k, P.rows), ns = ns_of(k, P.rows);
-          tot += ((nt + tpb - 1) / tpb) * ((ns + len - 1) / len);
-        }
-        return tot;
-      };
-      while (count(L) < want_items && L / 2 >= min_split(P.family)) L /= 2;
-    }
+    // Split length: the tuned one (or the longest source range), halved while
+    // the active instances give fewer items than want_items, so that a
+    // shrinking active set (instances converge one by one) still fills the
+    // device.
+    int64_t L = 0;
+    for (int32_t k : active) L = std::max(L, ns_of(k, P.rows));
+    if (b->tune.max_split > 0) L = std::min(L, b->tune.max_split);
+    auto count = [&](int64_t len) {
+      int64_t tot = 0;
+      for (int32_t k : active) {
+        const int64_t nt = nt_of(k, P.rows), ns = ns_of(k, P.rows);
+        tot += ((nt + tpb - 1) / tpb) * ((ns + len - 1) / len);
+      }
+      return tot;
+    };
+    while (count(L) < want_items && L / 2 >= min_split(P.family)) L /= 2;
     if (L < g) L = g;
     P.split = L;
     P.insts.assign(b->count, BatchInst{0, 0, 0, 1, 0});
