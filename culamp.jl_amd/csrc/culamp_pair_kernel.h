@@ -589,9 +589,9 @@ template <int NV>
 __device__ __forceinline__ void lse_lazy(TState<float>& st, const float (&v)[NV],
                                          const float (&wv)[NV], const float tq, float& tqm,
                                          double& dl) {
-  float loc = 0.0f;
+  float loc = exp2_fast(v[0]) * wv[0];
 #pragma unroll
-  for (int u = 0; u < NV; ++u) loc = fmaf(exp2_fast(v[u]), wv[u], loc);
+  for (int u = 1; u < NV; ++u) loc = fmaf(exp2_fast(v[u]), wv[u], loc);
   const double Sn = st.S + (double)loc;
   if (loc < 0x1p100f && Sn > 0x1p-90) {
     st.S = Sn;
@@ -639,9 +639,9 @@ __device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], cons
       const int sh = (int)fmaxf(st.m - mn, -2000.0f);
       st.S *= ldexp(1.0, sh);
       st.m = mn;
-      float loc = 0.0f;
+      float loc = exp2_fast(v[0] - mn) * wv[0];
 #pragma unroll
-      for (int u = 0; u < NV; ++u) loc = fmaf(exp2_fast(v[u] - mn), wv[u], loc);
+      for (int u = 1; u < NV; ++u) loc = fmaf(exp2_fast(v[u] - mn), wv[u], loc);
       st.S += (double)loc;
     } else {
       const double mn = fmax((double)st.m, (double)tm);
@@ -682,9 +682,11 @@ __device__ __forceinline__ void accumulate(TState<R>& st, const R (&v)[NV], cons
         }
       }
     } else {
-      float ls = 0.0f, lc = 0.0f;
+      // (the sums start from the first term: no addition of zero)
+      float ls = exp2_fast(v[0]) * wv[0];
+      float lc = NEEDC ? ls * Cv[0] : 0.0f;
 #pragma unroll
-      for (int u = 0; u < NV; ++u) {
+      for (int u = 1; u < NV; ++u) {
         const float e = exp2_fast(v[u]) * wv[u];
         ls += e;
         if constexpr (NEEDC) lc = fmaf(e, Cv[u], lc);
