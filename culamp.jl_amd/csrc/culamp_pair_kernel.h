@@ -1471,24 +1471,28 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
   // difference from the requested offset.  (kc == 0, an infinite
   // regularisation, keeps the offset out of the accumulator; so does a tiny
   // |kc|, whose tinit could overflow.)
+  //
+  // tvec is set once per work item: a rescale (lse_lazy_folded) moves tinit
+  // and eoff, and the later sub-tiles of that lane add eoff - ebuilt to their
+  // exponents (rescales are rare once the offset centres the exponents), so
+  // that the accumulator seed is loop-invariant and never copied.
   const bool fold = fabsf(kc) >= 0x1p-40f;
   float tinit = 0.0f, lo = -s2, hi = s2;
-  double eoff = 0.0;
-  f32x16_t tvec;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) tvec[r] = 0.0f;
+  double eoff = 0.0, ebuilt = 0.0;
   auto set_offset = [&](const float off) {
     tinit = off / kc;
     lo = tinit - s2;
     hi = tinit + s2;
     eoff = (double)kc * (double)tinit;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) tvec[r] = tinit;
   };
   if (fold) {
     set_offset(tq);
+    ebuilt = eoff;
     if constexpr (KIND == KIND_LSE) dl = eoff - (double)tq;  // (reference m = 0)
   }
+  f32x16_t tvec;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) tvec[r] = fold ? tinit : 0.0f;
 
   // lse_lazy with the offset in the accumulator: the sub-tile's exponents v
   // carry the effective offset eoff (S holds sum 2^(v_true + tq - m + dl),
@@ -1665,6 +1669,11 @@ __device__ __forceinline__ void mfma_body(const PassDesc& d, const Work& w, cons
         if (fold) {
 #pragma unroll
           for (int r = 0; r < 16; ++r) v[r] = fmaf(kc, acc[r], sa[r]);
+          if (eoff != ebuilt) {  // (after a rescale of this lane)
+            const float dv = (float)(eoff - ebuilt);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] += dv;
+          }
           lse_lazy_folded(v);
         } else {
 #pragma unroll
